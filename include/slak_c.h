@@ -1,0 +1,129 @@
+/* C-ABI of the MI355X Catmull-Clark library (libslak.so).
+ *
+ * Plain pointers and sizes only. Mesh pointers are device pointers. Every
+ * entry point returns a status code (SLAK_OK = 0); slak_last_error() gives the
+ * message of the calling thread's last failure. The C++ headers
+ * LAKCatmullClark.h / CatmullClark.h turn a failure into a throw of that
+ * message as `const char*`, the reference's error convention
+ * (LibInclude/cuda_host_helpers.h:34-41).
+ */
+#ifndef SLAK_C_H
+#define SLAK_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SLAK_OK = 0,
+  SLAK_ERR_INVALID_ARGUMENT = 1,
+  SLAK_ERR_INVALID_MESH = 2,
+  SLAK_ERR_HIP = 3,
+  SLAK_ERR_OUT_OF_MEMORY = 4,
+  SLAK_ERR_INTERNAL = 5,
+  SLAK_ERR_REJECTED = 6
+};
+
+/* MeshInfo::MeshType (LibInclude/LAKCatmullClark.h:98) */
+enum { SLAK_TRI = 0, SLAK_QUAD = 1, SLAK_POLY = 2 };
+
+/* LAKCatmullClark (sparse linear algebra) or CatmullClark (specialised). */
+enum { SLAK_SCHEME_LAK = 0, SLAK_SCHEME_SLAK = 1 };
+
+/* MeshInfo<int,int,int,float> (LibInclude/LAKCatmullClark.h:95-156,
+ * LibInclude/CatmullClark.h:69-130): face-vertex incidence matrix with one CSR
+ * row per face; vals[k] is the 1-based position of the corner in its face;
+ * verts holds float4 (x, y, z, w = 1) per vertex. All pointers on the device. */
+typedef struct slak_mesh {
+  int32_t type;
+  int32_t is_reduced;
+  uint64_t nfaces;
+  uint64_t nverts;
+  uint64_t nnz;
+  uint64_t max_face_order;
+  int32_t* ptr;
+  int32_t* ids;
+  int32_t* vals;
+  float* verts;
+} slak_mesh;
+
+/* Per-call report: replaces the reference's stdout banner
+ * (LAKCatmullClarkImpl.cu:560-570, CatmullClarkImpl.cu:504-514). */
+typedef struct slak_stats {
+  double wall_ms;       /* host wall clock of the whole call, inputs resident */
+  uint64_t peak_bytes;  /* peak device bytes held by the library's pool */
+  int32_t levels;       /* levels refined */
+  int32_t host_syncs;   /* blocking device->host round trips */
+  uint64_t out_faces;
+  uint64_t out_verts;
+} slak_stats;
+
+/* LAKCatmullClark::subdivideIteratively / CatmullClark::subdivideIteratively
+ * (LibInclude/LAKCatmullClark.h:190-191, LibInclude/CatmullClark.h:163-164;
+ * LibSource/device/LAKCatmullClarkImpl.cu:512-571,
+ * LibSource/device/CatmullClarkImpl.cu:457-515).
+ * A non-quad input always gets one level; quad levels follow up to
+ * target_level. On success rmesh receives newly allocated device buffers owned
+ * by the caller (free them with slak_free): LAK form has ptr/ids/vals
+ * (ptr[i] = 4i, vals cyclic 1..4, is_reduced = 0), SLAK form ids only
+ * (is_reduced = 1). A quad input with target_level 0 leaves rmesh untouched.
+ * The input is never modified. stats and stream may be NULL. Synchronous. */
+int slak_subdivide(int32_t scheme, const slak_mesh* cmesh, slak_mesh* rmesh, int32_t target_level,
+                   slak_stats* stats, void* stream);
+
+/* Frees a buffer returned by the library (replaces cudaFree in
+ * MeshInfo::freeAndReset, LibInclude/LAKCatmullClark.h:120-127). Like
+ * hipFree it waits for the device first. Also accepts slak_malloc pointers. */
+int slak_free(void* device_ptr);
+
+/* Message of this thread's last failure ("" if none). */
+const char* slak_last_error(void);
+
+/* Library version, e.g. 100 for 0.1.0. */
+int32_t slak_version(void);
+
+/* ---- device memory helpers for hosts without a GPU runtime binding ---- */
+int slak_malloc(void** device_ptr, uint64_t bytes);
+int slak_memcpy_h2d(void* dst_device, const void* src_host, uint64_t bytes);
+int slak_memcpy_d2h(void* dst_host, const void* src_device, uint64_t bytes);
+int slak_memcpy_d2d(void* dst_device, const void* src_device, uint64_t bytes);
+int slak_device_synchronize(void);
+/* Returns cached pool blocks to the device. */
+int slak_pool_trim(void);
+/* Bytes the pool holds in use / cached. */
+int slak_pool_stats(uint64_t* in_use, uint64_t* cached, uint64_t* peak);
+
+/* ---- per-stage timing (TimeProfileIndividual, LibInclude/profiling.h:182-200) ----
+ * slak_profile_enable(1) clears the record and brackets every later kernel
+ * launch with a HIP event pair on its stream; slak_profile_report writes
+ * {"stages": [{"name", "launches", "total_ms", "bytes"}]} as JSON text, where
+ * bytes are the stage's algorithmic (compulsory) bytes summed over launches. */
+int slak_profile_enable(int32_t on);
+int slak_profile_report(char* buf, uint64_t len);
+
+/* ---- sparse operators of the LAK method (LibInclude/math/bhblas.h:30-51) ---- */
+
+/* Mapped SpGEMM of one level (spgemm_mapped, LibInclude/device/math/spgemm.cuh:102-309,
+ * followed by compress, LibInclude/device/LAKCatmullClarkKernels.cuh:217-252):
+ * F column v lists w with value f+1 for the half-edge v->w of face f; D column
+ * v lists w with value f+1 for w->v; E column c lists r < c with value -id
+ * (interior) or +id (boundary), ids 1-based in (c, r) order. Rows ascending.
+ * Output arrays are newly allocated device buffers (free with slak_free):
+ * ptr arrays have nverts+1 entries. *nextern receives the boundary edge count.
+ * The input mesh is a level-input mesh as for slak_subdivide. */
+int slak_spgemm_mapped(const slak_mesh* cmesh, int32_t** F_ptr, int32_t** F_ids, int32_t** F_vals,
+                       int32_t** D_ptr, int32_t** D_ids, int32_t** D_vals, int32_t** E_ptr, int32_t** E_ids,
+                       int32_t** E_vals, uint64_t* nedges, uint64_t* nextern, void* stream);
+
+/* Exclusive prefix sum with total: out[i] = sum(in[0..i)), out[n] = sum(in)
+ * (CubFunctions::scan_exclusive, LibInclude/device/cub_wrappers.cuh:30-43,
+ * which also writes n+1 values). Device pointers; out holds n+1 int32. */
+int slak_scan_exclusive(const int32_t* in, int32_t* out, uint64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLAK_C_H */

@@ -1,0 +1,142 @@
+// Host-side declarations shared by the kernel launchers and the runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <exception>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <map>
+#include <vector>
+
+#include "slak_device.hpp"
+
+namespace slak {
+
+// ---------------------------------------------------------------------------
+// errors: every failure becomes a slak::Error carrying a C-ABI status code
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define SLAK_HIP_CHECK(expr)                                                                           \
+  do {                                                                                                 \
+    hipError_t _e = (expr);                                                                            \
+    if (_e != hipSuccess)                                                                              \
+      throw ::slak::Error(_e == hipErrorOutOfMemory ? 4 : 3,                                           \
+                          std::string("HIP error in ") + #expr + ": " + hipGetErrorString(_e));        \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// launchers (slak_kernels.hip)
+
+void launch_fill_i32(int32_t* p, int64_t n, int32_t value, hipStream_t st);
+void launch_validate(const int32_t* ptr, const int32_t* ids, const int32_t* vals, int32_t stride, int32_t nf,
+                     int32_t nv, int64_t nnz, bool require_quad, int32_t* ids_pos, int32_t* cface, int* flags,
+                     hipStream_t st);
+void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int32_t* fptr, const float4* P, int32_t nf,
+                  int32_t nv, float4* out, int32_t* cnt, hipStream_t st);
+void launch_transpose(bool quad, const int32_t* ids, const int32_t* fptr, int32_t nf, const int32_t* tptr,
+                      int32_t* cnt, int32_t* tids, hipStream_t st);
+void launch_symbolic(bool quad, int scheme, const LevelView& L, hipStream_t st);
+void launch_numeric(bool quad, int scheme, const LevelView& L, hipStream_t st);
+void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st);
+void launch_topology(bool quad, const int32_t* ids, const int32_t* fptr, const int32_t* eout, int32_t nv, int32_t nf,
+                     int64_t nnz, int32_t* rids, int32_t* rptr, int32_t* rvals, hipStream_t st);
+int64_t scan_partials_len(int64_t n);
+void launch_scan(const int32_t* in, int32_t* out, int64_t n, int64_t* partial, int* flags, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// caching device allocator (one per device)
+
+class Pool {
+ public:
+  void* alloc(size_t bytes);
+  // Returns the block to the cache. Callers free only after the stream that
+  // used the block has been synchronized.
+  bool release(void* p);
+  bool owns(void* p);
+  void trim();
+  size_t in_use() const { return in_use_; }
+  size_t cached() const { return cached_; }
+  size_t peak() const { return peak_; }
+  void reset_peak() { peak_ = in_use_; }
+
+ private:
+  std::mutex mu_;
+  std::multimap<size_t, void*> free_;
+  std::unordered_map<void*, size_t> used_;
+  size_t in_use_ = 0, cached_ = 0, peak_ = 0;
+};
+
+Pool& pool();
+
+// Allocations of one library call: released back to the pool after the
+// call's final stream synchronization, except buffers handed to the caller.
+class Arena {
+ public:
+  explicit Arena(Pool& p) : pool_(p) {}
+  // On an error path kernels of the call may still be queued: wait for the
+  // device before the blocks go back to the pool.
+  ~Arena() {
+    if (!blocks_.empty() && std::uncaught_exceptions() > 0) (void)hipDeviceSynchronize();
+    release_all();
+  }
+  void* get(size_t bytes);
+  template <class T>
+  T* get_n(int64_t n) {
+    return static_cast<T*>(get(sizeof(T) * (size_t)(n > 0 ? n : 1)));
+  }
+  void keep(void* p);  // ownership moves to the caller
+  void release_all();
+
+ private:
+  Pool& pool_;
+  std::vector<void*> blocks_;
+};
+
+// ---------------------------------------------------------------------------
+// per-stage HIP-event timing (TimeProfileIndividual, LibInclude/profiling.h:182-200):
+// when enabled, every launch site records an event pair on its stream and the
+// stage's algorithmic bytes (compulsory reads + writes of distinct arrays).
+
+class Profiler {
+ public:
+  static Profiler& get();
+  bool enabled() const { return enabled_; }
+  void set_enabled(bool on) { enabled_ = on; }
+  void begin(hipStream_t st, const char* name, double bytes);
+  void end(hipStream_t st);
+  void reset();
+  std::string report();  // JSON
+
+ private:
+  struct Rec {
+    const char* name;
+    double bytes;
+    hipEvent_t a, b;
+  };
+  hipEvent_t event();
+  std::mutex mu_;
+  bool enabled_ = false;
+  std::vector<Rec> recs_;
+  std::vector<hipEvent_t> spare_;
+};
+
+struct ProfScope {
+  hipStream_t st;
+  bool on;
+  ProfScope(hipStream_t s, const char* name, double bytes) : st(s), on(Profiler::get().enabled()) {
+    if (on) Profiler::get().begin(st, name, bytes);
+  }
+  ~ProfScope() {
+    if (on) Profiler::get().end(st);
+  }
+};
+
+}  // namespace slak

@@ -1,0 +1,251 @@
+"""Parity of the HIP path (through the C-ABI) against the CPU oracle.
+
+Bar: refined topology bit-exact against the oracle's canonical LAK / SLAK
+numbering; FP32 positions within 1e-5 of the control cage's bounding-box
+diagonal of the FP64 oracle (POS_TOL below)."""
+import numpy as np
+import pytest
+
+from slak_amd import capi, meshgen as g
+from slak_amd.mesh import MeshInfo, LAKCatmullClark, CatmullClark, subdivide_host, spgemm_mapped_host
+from slak_amd.capi import SlakError
+from tests import refimpl as R
+
+pytestmark = pytest.mark.gpu
+
+LAK, SLAK = capi.SLAK_SCHEME_LAK, capi.SLAK_SCHEME_SLAK
+POS_TOL = 1e-5  # max |p_gpu - p_fp64| / bbox diagonal
+
+
+def rel_err(gpu_verts, oracle_P, diag):
+    return float(np.abs(gpu_verts.astype(np.float64) - oracle_P).max() / diag)
+
+
+def check_against_oracle(oracle, mesh, levels, scheme):
+    out, st = subdivide_host(mesh, levels, scheme)
+    ref = oracle.subdivide(mesh, levels, scheme=scheme, precision=64)
+    assert out.nfaces == ref.nf and out.nverts == ref.nv
+    np.testing.assert_array_equal(out.ids, ref.ids)
+    if scheme == LAK:
+        np.testing.assert_array_equal(out.ptr, ref.ptr)
+        np.testing.assert_array_equal(out.vals, ref.vals)
+    else:
+        assert out.ptr is None and out.vals is None
+    err = rel_err(out.verts, ref.P, mesh.bbox_diag())
+    assert err <= POS_TOL, err
+    assert np.all(out.verts[:, 3] == 1.0)
+    return out, ref, err
+
+
+SMALL = [
+    ("cube", lambda: g.cube()),
+    ("tetrahedron", lambda: g.tetrahedron()),
+    ("quad", lambda: g.single_quad()),
+    ("open_grid", lambda: g.open_grid(5, 3, sigma=0.01)),
+    ("cylinder", lambda: g.cylinder(7, 3)),
+    ("mixed_pent", lambda: g.torus_mixed(12, 8, 0.3, pentagons=True, seed=7)),
+    ("mixed_tri", lambda: g.torus_mixed(9, 7, 0.5, seed=3)),
+]
+
+
+@pytest.mark.parametrize("name,make", SMALL, ids=[s[0] for s in SMALL])
+@pytest.mark.parametrize("scheme", [LAK, SLAK], ids=["LAK", "SLAK"])
+@pytest.mark.parametrize("levels", [1, 2, 3])
+def test_small_meshes(oracle, name, make, scheme, levels):
+    check_against_oracle(oracle, make(), levels, scheme)
+
+
+@pytest.mark.parametrize("scheme", [LAK, SLAK], ids=["LAK", "SLAK"])
+def test_config1_torus_3_levels(oracle, scheme):
+    _, _, err = check_against_oracle(oracle, g.config1(), 3, scheme)
+    print(f"cfg1 scheme={scheme} max_err/diag={err:.3e}")
+
+
+@pytest.mark.parametrize("scheme", [LAK, SLAK], ids=["LAK", "SLAK"])
+def test_config2_mixed_3_levels(oracle, scheme):
+    _, _, err = check_against_oracle(oracle, g.config2(), 3, scheme)
+    print(f"cfg2 (3 levels) scheme={scheme} max_err/diag={err:.3e}")
+
+
+def _iterate(mesh, levels, scheme):
+    cur = mesh
+    for _ in range(levels):
+        cur, _ = subdivide_host(cur, 1, scheme)
+    return cur
+
+
+@pytest.mark.parametrize("scheme", [LAK, SLAK], ids=["LAK", "SLAK"])
+def test_config2_4_levels_chain_equals_iterated(scheme):
+    """Full-size cfg2: the 4-level chain (closed-form sizes, fused kernels)
+    equals four single-level calls bit for bit; counts follow the closed form."""
+    m = g.config2()
+    out, st = subdivide_host(m, 4, scheme)
+    it = _iterate(m, 4, scheme)
+    np.testing.assert_array_equal(out.ids, it.ids)
+    np.testing.assert_array_equal(out.verts, it.verts)
+    assert out.nfaces == m.nnz * 4 ** 3
+    assert st.levels == 4
+
+
+def test_lak_slak_positions_identical_up_to_permutation():
+    m = g.torus_mixed(40, 24, 0.25, pentagons=True, seed=9)
+    a, _ = subdivide_host(m, 3, LAK)
+    b, _ = subdivide_host(m, 3, SLAK)
+    perm = np.full(b.nverts, -1, dtype=np.int64)
+    perm[b.ids] = a.ids
+    assert (perm >= 0).all()
+    assert len(np.unique(perm)) == b.nverts
+    np.testing.assert_array_equal(b.verts, a.verts[perm])
+
+
+def test_deterministic_bitwise():
+    m = g.torus_mixed(64, 32, 0.25, pentagons=True, seed=5)
+    a, _ = subdivide_host(m, 3, LAK)
+    b, _ = subdivide_host(m, 3, LAK)
+    np.testing.assert_array_equal(a.ids, b.ids)
+    np.testing.assert_array_equal(a.verts.view(np.uint32), b.verts.view(np.uint32))
+
+
+def test_config3_bspline_and_chain():
+    """1024 x 1024 torus, 2 levels: every refined position equals the FP64
+    tensor-product B-spline refinement of the control grid (CC on a regular
+    closed grid), and the 2-level chain equals two single-level calls."""
+    m = g.config3()
+    a = b = 1024
+    full, st = subdivide_host(m, 2, LAK)
+    assert full.nfaces == 16 * a * b
+    coords = R.torus_coords(a, b)
+    quads = m.ids.reshape(-1, 4)
+    G = np.zeros((a, b, 3))
+    G[coords[:, 0], coords[:, 1]] = m.verts[:, :3]
+    cur, A, B = m, a, b
+    diag = m.bbox_diag()
+    for level in range(2):
+        out, _ = subdivide_host(cur, 1, LAK)
+        nc = R.refine_grid_coords(coords, quads, out.ids, A, B, out.nverts)
+        assert (nc >= 0).all()
+        G = R.bspline_refine(G)
+        A, B = 2 * A, 2 * B
+        err = float(np.abs(out.verts[:, :3] - G[nc[:, 0], nc[:, 1]]).max()) / diag
+        print(f"cfg3 level {level + 1}: max_err/diag={err:.3e}")
+        assert err <= POS_TOL
+        G[nc[:, 0], nc[:, 1]] = out.verts[:, :3].astype(np.float64)
+        coords, quads, cur = nc, out.ids.reshape(-1, 4), out
+    np.testing.assert_array_equal(full.ids, cur.ids)
+    np.testing.assert_array_equal(full.verts, cur.verts)
+
+
+# ---------------------------------------------------------------------------
+# operators
+
+
+@pytest.mark.parametrize("make", [g.config1, lambda: g.torus_mixed(16, 10, 0.3, pentagons=True, seed=4),
+                                  lambda: g.open_grid(6, 4), g.cube],
+                         ids=["cfg1", "mixed", "open_grid", "cube"])
+def test_spgemm_operator_matches_oracle(oracle, make):
+    m = make()
+    r = spgemm_mapped_host(m)
+    d = oracle.subdivide(m, 1, scheme=0, keep_debug=True).debug[0]
+    for name in ("F", "D", "E"):
+        for part in ("ptr", "ids", "vals"):
+            np.testing.assert_array_equal(getattr(r, f"{name}_{part}"), d[f"{name}_{part}"], err_msg=f"{name}_{part}")
+    assert r.nedges == d["nedges"] and r.nextern == d["nextern"]
+
+
+@pytest.mark.parametrize("n", [0, 1, 7, 2047, 2048, 2049, 100000, 3_000_001])
+def test_scan_operator(n):
+    import ctypes
+    from slak_amd.mesh import DeviceBuffer, to_host
+    rng = np.random.default_rng(n)
+    x = rng.integers(0, 9, size=n).astype(np.int32)
+    din = DeviceBuffer.from_array(x if n else np.zeros(1, np.int32))
+    dout = DeviceBuffer(4 * (n + 1))
+    capi.check(capi.lib().slak_scan_exclusive(din.ptr, dout.ptr, n, None))
+    y = to_host(dout.ptr, n + 1, np.int32)
+    ref = np.concatenate([[0], np.cumsum(x, dtype=np.int64)]).astype(np.int32)
+    np.testing.assert_array_equal(y, ref)
+
+
+# ---------------------------------------------------------------------------
+# boundary behaviour of the API
+
+
+def test_vals_permutation_respected_by_lak(oracle):
+    """LAK puts corners in vals order (the reference sorts each face by its
+    values before refining); SLAK never reads vals and uses storage order."""
+    m = g.torus_mixed(10, 6, 0.3, pentagons=True, seed=3)
+    rng = np.random.default_rng(1)
+    ids2, vals2 = m.ids.copy(), m.vals.copy()
+    ids3 = m.ids.copy()
+    for f in range(m.nfaces):
+        s, e = m.ptr[f], m.ptr[f + 1]
+        p = rng.permutation(e - s)
+        ids2[s:e], vals2[s:e] = m.ids[s:e][p], m.vals[s:e][p]
+        ids3[s:e] = np.roll(m.ids[s:e], int(rng.integers(0, e - s)))  # same polygon, other first corner
+    m2 = g.HostMesh(m.type, m.ptr, ids2, vals2, m.verts)
+    a, _ = subdivide_host(m, 2, LAK)
+    b, _ = subdivide_host(m2, 2, LAK)
+    np.testing.assert_array_equal(a.ids, b.ids)
+    np.testing.assert_array_equal(a.verts, b.verts)
+    # storage order with scrambled vals: SLAK refines the rotated faces as stored
+    m3 = g.HostMesh(m.type, m.ptr, ids3, vals2, m.verts)
+    c, _ = subdivide_host(m3, 1, SLAK)
+    ref = oracle.subdivide(m3, 1, scheme=SLAK)
+    np.testing.assert_array_equal(c.ids, ref.ids)
+
+
+def test_input_not_modified_and_level_zero():
+    m = g.torus(8, 6)
+    c = MeshInfo.from_host(m)
+    r = MeshInfo()
+    try:
+        st = LAKCatmullClark.subdivideIteratively(c, r, 0)  # quad input, level 0: untouched
+        assert st.levels == 0 and r.ids == 0
+        LAKCatmullClark.subdivideIteratively(c, r, 2)
+        back = c.to_host()
+        np.testing.assert_array_equal(back.ids, m.ids)
+        np.testing.assert_array_equal(back.vals, m.vals)
+        np.testing.assert_array_equal(back.verts, m.verts)
+        assert r.nfaces == 8 * 6 * 16 and not r.is_reduced
+    finally:
+        c.freeAndReset()
+        r.freeAndReset()
+    # a non-quad input always gets one level
+    t = g.tetrahedron()
+    c = MeshInfo.from_host(t)
+    r = MeshInfo()
+    try:
+        st = CatmullClark.subdivideIteratively(c, r, 0)
+        assert st.levels == 1 and r.nfaces == 12 and r.is_reduced and r.ptr == 0
+    finally:
+        c.freeAndReset()
+        r.freeAndReset()
+
+
+def _corrupt(kind):
+    m = g.torus_mixed(6, 5, 0.3, pentagons=True, seed=1)
+    ptr, ids, vals = m.ptr.copy(), m.ids.copy(), m.vals.copy()
+    if kind == "bad_id":
+        ids[5] = m.nverts + 3
+    elif kind == "negative_id":
+        ids[0] = -1
+    elif kind == "bad_vals":
+        vals[1] = vals[0]
+    elif kind == "bad_ptr":
+        ptr[3] = ptr[4] + 1
+    elif kind == "duplicate_face":
+        ptr = np.concatenate([ptr, [ptr[-1] + (ptr[1] - ptr[0])]])
+        ids = np.concatenate([ids, ids[ptr[0]:ptr[1]]])
+        vals = np.concatenate([vals, vals[ptr[0]:ptr[1]]])
+    return g.HostMesh(g.POLY, ptr.astype(np.int32), ids.astype(np.int32), vals.astype(np.int32), m.verts)
+
+
+@pytest.mark.parametrize("kind", ["bad_id", "negative_id", "bad_vals", "bad_ptr", "duplicate_face"])
+def test_invalid_mesh_rejected(kind):
+    bad = _corrupt(kind)
+    with pytest.raises(SlakError) as ei:
+        subdivide_host(bad, 1, LAK)
+    assert ei.value.code == 2, str(ei.value)
+    # the library stays usable after a rejected call
+    ok, _ = subdivide_host(g.cube(), 1, LAK)
+    assert ok.nfaces == 24
