@@ -122,6 +122,37 @@ int slak_spgemm_mapped(const slak_mesh* cmesh, int32_t** F_ptr, int32_t** F_ids,
  * which also writes n+1 values). Device pointers; out holds n+1 int32. */
 int slak_scan_exclusive(const int32_t* in, int32_t* out, uint64_t n, void* stream);
 
+/* ---- batched subdivision (no counterpart in the reference, which refines
+ * one mesh per call) ----
+ * slak_batch_create validates the meshes (device pointers, kept by pointer:
+ * they must stay valid and unchanged in topology until destroy), runs the
+ * first level's analysis for all of them (two blocking round trips) and
+ * allocates every buffer of the whole chain, sized in closed form.
+ * slak_batch_run recomputes the full chain from the current inputs (positions
+ * may change between runs) with one launch per kernel per level over the union
+ * of the meshes still active at that level; use_graph = 1 replays it as one
+ * hipGraph captured on the first run. Synchronous.
+ * slak_batch_result gives a view of mesh `index`'s refined mesh in the form
+ * slak_subdivide returns; the buffers belong to the batch (do not free). */
+typedef struct slak_batch slak_batch;
+
+typedef struct slak_batch_info {
+  int32_t nmeshes;
+  int32_t max_levels;
+  int32_t host_syncs;
+  int32_t reserved;
+  uint64_t device_bytes;
+  uint64_t out_faces;
+  uint64_t out_verts;
+} slak_batch_info;
+
+int slak_batch_create(int32_t scheme, int32_t nmeshes, const slak_mesh* meshes, const int32_t* target_levels,
+                      slak_batch** batch, void* stream);
+int slak_batch_run(slak_batch* batch, int32_t use_graph);
+int slak_batch_result(slak_batch* batch, int32_t index, slak_mesh* rmesh);
+int slak_batch_info_get(slak_batch* batch, slak_batch_info* info);
+int slak_batch_destroy(slak_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,7 +17,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libslak.so")
 OBJDIR = os.path.join(ROOT, "_build", "slak")
 
-SOURCES = ["slak_kernels.hip", "slak_runtime.hip"]
+SOURCES = ["slak_kernels.hip", "slak_runtime.hip", "slak_plan.hip", "slak_batch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [
@@ -57,6 +57,14 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # C++ program that uses the drop-in headers against the library
+    dropin = os.path.join(LIBDIR, "slak_dropin_test")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "dropin_test.cpp"), "-L", LIBDIR, "-lslak",
+           "-Wl,-rpath,$ORIGIN", "-o", dropin]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"drop-in test build failed:\n{r.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB

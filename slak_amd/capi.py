@@ -48,6 +48,18 @@ class SlakStats(ctypes.Structure):
     ]
 
 
+class SlakBatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("nmeshes", ctypes.c_int32),
+        ("max_levels", ctypes.c_int32),
+        ("host_syncs", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("device_bytes", ctypes.c_uint64),
+        ("out_faces", ctypes.c_uint64),
+        ("out_verts", ctypes.c_uint64),
+    ]
+
+
 class SlakError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"slak error {code}: {msg}")
@@ -73,6 +85,13 @@ EXPORTS = [
     ("slak_profile_enable", ctypes.c_int, [ctypes.c_int32]),
     ("slak_profile_report", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64]),
     ("slak_scan_exclusive", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
+    ("slak_batch_create", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(SlakMesh),
+                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_void_p),
+                                         ctypes.c_void_p]),
+    ("slak_batch_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("slak_batch_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(SlakMesh)]),
+    ("slak_batch_info_get", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SlakBatchInfo)]),
+    ("slak_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
 ]
 
 _lib: Optional[ctypes.CDLL] = None

@@ -1,13 +1,18 @@
 // Device-side building blocks of the Catmull-Clark level pipeline (gfx950).
 //
-// Data layout in HBM (one level, all int32 unless noted):
+// A level runs over a union of meshes ("segments"): mesh k owns the vertex
+// range [vb_k, vb_k + nv_k), faces [fb_k, ...) and corners [cb_k, ...) of the
+// level's arrays, which hold global (union) indices. Data layout in HBM:
 //   ids[nnz]           position-ordered corner -> vertex (quad levels: 4 per face, int4-aligned)
 //   fptr[nf+1], cface[nnz]   face offsets and corner -> face (poly levels only)
 //   P[nv]              float4 positions (x, y, z, w=1)
 //   tptr[nv+1], tids[nnz]    M^T: vertex -> incident corners (corner index t)
-//   eptr[nv+1]         per-vertex offset of the edges the vertex owns
-//   eout[nnz]          edge id of every corner's outgoing half-edge
-//   out[nv+nf+ne]      float4 refined positions [vertex points | face points | edge points]
+//   eptr[nv+1]         per-vertex offset of the edges the vertex owns (global scan)
+//   eout[nnz]          mesh-local edge id of every corner's outgoing half-edge
+// Outputs go to each segment's own destination: refined positions
+// [vertex points | face points | edge points] and refined quads in the
+// mesh's local numbering (+ the segment's id offset when the result feeds the
+// next level's union).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,6 +34,48 @@ enum : int {
   kFlagOverflow = 128,
 };
 
+// One mesh of a level's union.
+struct SegDesc {
+  int64_t vb, fb, cb;      // union bases: vertices, faces, corners
+  int32_t nv, nf;          // local sizes
+  int64_t nnz;             // local corners
+  int64_t ne;              // local edges (edge points the destination holds)
+  float4* dst_verts;       // nv + nf + ne refined positions
+  int32_t* dst_ids;        // 4 * nnz refined quad ids
+  int32_t* dst_ptr;        // LAK form (final level) or null
+  int32_t* dst_vals;       // LAK form (final level) or null
+  int64_t id_offset;       // added to local refined ids
+  // first-level sources (the caller's mesh)
+  const int32_t* src_ptr;  // null: homogeneous with src_stride
+  const int32_t* src_ids;
+  const int32_t* src_vals; // null: storage order
+  const float4* src_verts;
+  int32_t src_stride;
+  int32_t require_quad;
+};
+
+struct SegTable {
+  const SegDesc* d;
+  const int64_t* vbase;  // n + 1 vertex bases
+  const int64_t* fbase;  // n + 1 face bases
+  int32_t n;
+  int* mflags;           // per-segment validation flags
+};
+
+// Segment of a union index: the last k with base[k] <= x (empty segments skipped).
+__device__ __forceinline__ int seg_of(const int64_t* base, int n, int64_t x) {
+  if (n <= 1) return 0;
+  int lo = 0, hi = n;  // base[lo] <= x < base[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (base[mid] <= x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 struct LevelView {
   const int32_t* ids;
   const int32_t* fptr;   // poly levels
@@ -42,16 +89,16 @@ struct LevelView {
   int32_t* ecnt;
   const int32_t* eptr;
   int32_t* eout;
-  int32_t* E_ids;   // optional LAK edge matrix rows
+  int32_t* E_ids;   // optional LAK edge matrix rows (single segment)
   int32_t* E_vals;  // optional LAK edge matrix values (+-(id+1))
-  float4* out;
   int* flags;
-  int64_t ne_cap;   // edge points the output holds
+  SegTable seg;
   // mapped SpGEMM operator outputs (F, D rows share M^T's row offsets)
   int32_t* F_ids;
   int32_t* F_vals;
   int32_t* D_ids;
   int32_t* D_vals;
+  int64_t ne_cap;  // E capacity of the SpGEMM operator
 };
 
 __device__ __forceinline__ float4 f4add(float4 a, float4 b) {
@@ -298,19 +345,23 @@ __device__ __forceinline__ void row_numeric(Row& r, const LevelView& L, int32_t 
   constexpr int C = Row::kCap;
   // local copies: lambdas below capture locals only, so the kernel argument
   // block never has its address taken (it would be spilled to scratch)
+  const int k = seg_of(L.seg.vbase, L.seg.n, v);
+  const SegDesc* __restrict__ S = L.seg.d + k;
+  const int64_t vb = S->vb, fb = S->fb;
+  const int32_t snv = S->nv, snf = S->nf;
+  const int64_t ne_cap = S->ne;
+  float4* __restrict__ dst = S->dst_verts;
   const float4* __restrict__ P = L.P;
-  const float4* __restrict__ fp = L.out + L.nv;
-  float4* __restrict__ ep = L.out + (int64_t)L.nv + L.nf;
-  float4* __restrict__ vout = L.out;
+  auto FP = [dst, snv, fb](int32_t f) __attribute__((always_inline)) { return dst[snv + (f - fb)]; };
+  float4* __restrict__ ep = dst + snv + snf;
   int32_t* __restrict__ eout = L.eout;
   int32_t* __restrict__ E_ids = L.E_ids;
   int32_t* __restrict__ E_vals = L.E_vals;
   int* flags = L.flags;
-  const int64_t ne_cap = L.ne_cap;
   const float4 Pv = P[v];
-  const int32_t e0 = L.eptr[v];
+  const int32_t e0 = L.eptr[v] - L.eptr[vb];  // mesh-local edge base of v
   const int n = r.n;
-  // Edge ids beyond the allocated edge points are clamped (and flagged) so
+  // Edge ids beyond the segment's edge points are clamped (and flagged) so
   // that refined ids stay inside the refined vertex range.
   auto cap = [flags, ne_cap](int32_t e) __attribute__((always_inline)) {
     if (e >= ne_cap) {
@@ -329,7 +380,7 @@ __device__ __forceinline__ void row_numeric(Row& r, const LevelView& L, int32_t 
         int jm = r.JM(i);
         float4 q;
         if (jm >= 0)  // interior: f0 = face of x->v, f1 = face of v->x
-          q = f4mul(f4add(f4add(f4add(P[x], Pv), fp[r.FJ(i)]), fp[r.F(i)]), 0.25f);
+          q = f4mul(f4add(f4add(f4add(P[x], Pv), FP(r.FJ(i))), FP(r.F(i))), 0.25f);
         else
           q = f4mul(f4add(P[x], Pv), 0.5f);
         if (e < ne_cap) ep[e] = q;
@@ -352,16 +403,16 @@ __device__ __forceinline__ void row_numeric(Row& r, const LevelView& L, int32_t 
       }
     });
   } else {
-    int k = 0;
+    int kk = 0;
     rloop<C>(n, [&](int i) __attribute__((always_inline)) {
       int32_t x = r.W(i);
       int jm = r.JM(i);
       if (jm < 0 || x > v) {
-        int32_t e = cap(e0 + k);
-        ++k;
+        int32_t e = cap(e0 + kk);
+        ++kk;
         float4 q;
         if (jm >= 0)  // interior, v < x: f0 = face of v->x, f1 = face of x->v
-          q = f4mul(f4add(f4add(f4add(Pv, P[x]), fp[r.F(i)]), fp[r.FJ(i)]), 0.25f);
+          q = f4mul(f4add(f4add(f4add(Pv, P[x]), FP(r.F(i))), FP(r.FJ(i))), 0.25f);
         else
           q = (v < x) ? f4mul(f4add(Pv, P[x]), 0.5f) : f4mul(f4add(P[x], Pv), 0.5f);
         if (e < ne_cap) ep[e] = q;
@@ -377,7 +428,7 @@ __device__ __forceinline__ void row_numeric(Row& r, const LevelView& L, int32_t 
   float4 q;
   if (!bnd) {
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-    rloop<C>(n, [&](int i) __attribute__((always_inline)) { s = f4add(s, fp[r.F(i)]); });
+    rloop<C>(n, [&](int i) __attribute__((always_inline)) { s = f4add(s, FP(r.F(i))); });
     rloop<C>(n, [&](int j) __attribute__((always_inline)) { s = f4add(s, P[r.U(j)]); });
     float nf = (float)n;
     q = f4div(f4add(f4div(s, nf), f4mul(Pv, nf - 2.0f)), nf);
@@ -390,7 +441,7 @@ __device__ __forceinline__ void row_numeric(Row& r, const LevelView& L, int32_t 
       if (r.IM(j) < 0) q = f4add(q, f4mul(P[r.U(j)], 0.125f));
     });
   }
-  vout[v] = q;
+  dst[v - vb] = q;
 }
 
 // Mapped SpGEMM operator for row v (spgemm_mapped + compress): F row =

@@ -36,18 +36,19 @@ struct Error : std::runtime_error {
 // launchers (slak_kernels.hip)
 
 void launch_fill_i32(int32_t* p, int64_t n, int32_t value, hipStream_t st);
-void launch_validate(const int32_t* ptr, const int32_t* ids, const int32_t* vals, int32_t stride, int32_t nf,
-                     int32_t nv, int64_t nnz, bool require_quad, int32_t* ids_pos, int32_t* cface, int* flags,
-                     hipStream_t st);
+void launch_gather(const SegTable& seg, int32_t nf_total, bool use_vals, int32_t* ids_pos, int32_t* fptr,
+                   int32_t* cface, int* flags, hipStream_t st);
+void launch_gather_verts(const SegTable& seg, int32_t nv_total, float4* P, hipStream_t st);
+void launch_seg_bounds(const int64_t* vbase, int32_t n, const int32_t* eptr, int32_t* out, hipStream_t st);
 void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int32_t* fptr, const float4* P, int32_t nf,
-                  int32_t nv, float4* out, int32_t* cnt, hipStream_t st);
+                  const SegTable& seg, int32_t* cnt, hipStream_t st);
 void launch_transpose(bool quad, const int32_t* ids, const int32_t* fptr, int32_t nf, const int32_t* tptr,
                       int32_t* cnt, int32_t* tids, hipStream_t st);
 void launch_symbolic(bool quad, int scheme, const LevelView& L, hipStream_t st);
 void launch_numeric(bool quad, int scheme, const LevelView& L, hipStream_t st);
 void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st);
-void launch_topology(bool quad, const int32_t* ids, const int32_t* fptr, const int32_t* eout, int32_t nv, int32_t nf,
-                     int64_t nnz, int32_t* rids, int32_t* rptr, int32_t* rvals, hipStream_t st);
+void launch_topology(bool quad, const int32_t* ids, const int32_t* fptr, const int32_t* eout, int32_t nf,
+                     const SegTable& seg, hipStream_t st);
 int64_t scan_partials_len(int64_t n);
 void launch_scan(const int32_t* in, int32_t* out, int64_t n, int64_t* partial, int* flags, hipStream_t st);
 
@@ -94,10 +95,13 @@ class Arena {
   }
   void keep(void* p);  // ownership moves to the caller
   void release_all();
+  bool empty() const { return blocks_.empty(); }
+  size_t bytes() const { return bytes_; }
 
  private:
   Pool& pool_;
   std::vector<void*> blocks_;
+  size_t bytes_ = 0;
 };
 
 // ---------------------------------------------------------------------------

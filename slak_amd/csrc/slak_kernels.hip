@@ -42,34 +42,44 @@ __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t value) {
 }
 
 // ---------------------------------------------------------------------------
-// first-level validation + position ordering
+// first level: validate each caller mesh and gather it into the union in
+// position order (sortKeyValuePairsSegmentedInPlace, LAKCatmullClarkKernels.cuh:147-180)
 
-__global__ void k_validate(const int32_t* __restrict__ ptr, const int32_t* __restrict__ ids,
-                           const int32_t* __restrict__ vals, int32_t stride, int32_t nf, int32_t nv, int64_t nnz,
-                           int require_quad, int32_t* __restrict__ ids_pos, int32_t* __restrict__ cface,
-                           int* __restrict__ flags) {
+__global__ void k_gather(SegTable seg, int32_t nf_total, int use_vals, int32_t* __restrict__ ids_pos,
+                         int32_t* __restrict__ fptr, int32_t* __restrict__ cface, int* __restrict__ flags) {
   int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
-  int64_t s = ptr ? ptr[f] : (int64_t)stride * f;
-  int64_t e = ptr ? ptr[f + 1] : (int64_t)stride * (f + 1);
+  if (f >= nf_total) return;
+  const int k = seg_of(seg.fbase, seg.n, f);
+  const SegDesc* S = seg.d + k;
+  const int32_t lf = (int32_t)(f - S->fb);
+  const int32_t* sp = S->src_ptr;
+  const int32_t* sid = S->src_ids;
+  const int32_t* sval = use_vals ? S->src_vals : nullptr;
+  const int64_t snnz = S->nnz;
+  int64_t s = sp ? sp[lf] : (int64_t)S->src_stride * lf;
+  int64_t e = sp ? sp[lf + 1] : (int64_t)S->src_stride * (lf + 1);
   int bad = 0;
-  if (f == 0 && s != 0) bad |= kFlagBadPtr;
-  if (f == nf - 1 && e != nnz) bad |= kFlagBadPtr;
-  if (s < 0 || e < s || e > nnz) {
-    atomicOr(flags, bad | kFlagBadPtr);
+  if (lf == 0 && s != 0) bad |= kFlagBadPtr;
+  if (lf == S->nf - 1 && e != snnz) bad |= kFlagBadPtr;
+  if (s < 0 || e < s || e > snnz) {
+    bad |= kFlagBadPtr;
+    atomicOr(seg.mflags + k, bad);
+    atomicOr(flags, bad);
     return;
   }
-  int32_t order = (int32_t)(e - s);
+  const int32_t order = (int32_t)(e - s);
   if (order < 3) bad |= kFlagBadOrder;
-  if (require_quad && (order != 4 || s != 4ll * f)) bad |= kFlagBadQuad;
+  if (S->require_quad && (order != 4 || s != 4ll * lf)) bad |= kFlagBadQuad;
+  const int64_t cb = S->cb;
+  const int32_t vb = (int32_t)S->vb;
   uint32_t mask = 0;
-  for (int64_t k = s; k < e; ++k) {
-    int32_t id = ids[k];
-    if (id < 0 || id >= nv) {
+  for (int64_t c = s; c < e; ++c) {
+    int32_t id = sid[c];
+    if (id < 0 || id >= S->nv) {
       bad |= kFlagBadId;
       id = 0;
     }
-    int32_t pos = vals ? vals[k] : (int32_t)(k - s + 1);
+    int32_t pos = sval ? sval[c] : (int32_t)(c - s + 1);
     if (pos < 1 || pos > order) {
       bad |= kFlagBadVals;
       continue;
@@ -79,29 +89,54 @@ __global__ void k_validate(const int32_t* __restrict__ ptr, const int32_t* __res
       if (mask & bit) bad |= kFlagBadVals;
       mask |= bit;
     } else {
-      for (int64_t k2 = s; k2 < k; ++k2)
-        if ((vals ? vals[k2] : (int32_t)(k2 - s + 1)) == pos) bad |= kFlagBadVals;
+      for (int64_t c2 = s; c2 < c; ++c2)
+        if ((sval ? sval[c2] : (int32_t)(c2 - s + 1)) == pos) bad |= kFlagBadVals;
     }
-    ids_pos[s + pos - 1] = id;
-    if (cface) cface[k] = f;
+    ids_pos[cb + s + pos - 1] = vb + id;
+    if (cface) cface[cb + c] = f;
   }
-  if (bad) atomicOr(flags, bad);
+  if (fptr) {
+    fptr[f] = (int32_t)(cb + s);
+    if (f == nf_total - 1) fptr[nf_total] = (int32_t)(cb + e);
+  }
+  if (bad) {
+    atomicOr(seg.mflags + k, bad);
+    atomicOr(flags, bad);
+  }
+}
+
+__global__ void k_gather_verts(SegTable seg, int32_t nv_total, float4* __restrict__ P) {
+  int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv_total) return;
+  const int k = seg_of(seg.vbase, seg.n, v);
+  const SegDesc* S = seg.d + k;
+  P[v] = S->src_verts[v - S->vb];
+}
+
+// eptr at every segment's first vertex: out[k] = eptr[vbase[k]], k = 0..n
+__global__ void k_seg_bounds(const int64_t* __restrict__ vbase, int32_t n, const int32_t* __restrict__ eptr,
+                             int32_t* __restrict__ out) {
+  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > n) return;
+  out[k] = eptr[vbase[k]];
 }
 
 // ---------------------------------------------------------------------------
-// valence count (+ face points when the output buffer exists)
+// valence count (+ face points into each segment's destination)
 
 template <bool QUAD, bool FP, bool COUNT>
 __global__ void k_faces(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr, const float4* __restrict__ P,
-                        int32_t nf, int32_t nv, float4* __restrict__ out, int32_t* __restrict__ cnt) {
+                        int32_t nf, SegTable seg, int32_t* __restrict__ cnt) {
   int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
+  float4* dst = nullptr;
+  if (FP) {
+    const SegDesc* S = seg.d + seg_of(seg.fbase, seg.n, f);
+    dst = S->dst_verts + S->nv + (f - S->fb);
+  }
   if (QUAD) {
     int4 c = reinterpret_cast<const int4*>(ids)[f];
-    if (FP) {
-      float4 s = f4add(f4add(f4add(P[c.x], P[c.y]), P[c.z]), P[c.w]);
-      out[(int64_t)nv + f] = f4div(s, 4.0f);
-    }
+    if (FP) *dst = f4div(f4add(f4add(f4add(P[c.x], P[c.y]), P[c.z]), P[c.w]), 4.0f);
     if (COUNT) {
       atomicAdd(&cnt[c.x], 1);
       atomicAdd(&cnt[c.y], 1);
@@ -111,12 +146,12 @@ __global__ void k_faces(const int32_t* __restrict__ ids, const int32_t* __restri
   } else {
     int32_t s = fptr[f], e = fptr[f + 1];
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int32_t k = s; k < e; ++k) {
-      int32_t v = ids[k];
+    for (int32_t c = s; c < e; ++c) {
+      int32_t v = ids[c];
       if (FP) acc = f4add(acc, P[v]);
       if (COUNT) atomicAdd(&cnt[v], 1);
     }
-    if (FP) out[(int64_t)nv + f] = f4div(acc, (float)(e - s));
+    if (FP) *dst = f4div(acc, (float)(e - s));
   }
 }
 
@@ -176,7 +211,10 @@ __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
     row_symbolic<SCHEME>(r, v, count, bad);
   }
   L.ecnt[v] = count;
-  if (bad) atomicOr(L.flags, bad);
+  if (bad) {
+    atomicOr(L.flags, bad);
+    if (L.seg.mflags) atomicOr(L.seg.mflags + seg_of(L.seg.vbase, L.seg.n, v), bad);
+  }
 }
 
 template <bool QUAD, int SCHEME>
@@ -212,29 +250,37 @@ __global__ void __launch_bounds__(kBlock) k_spgemm_fde(LevelView L) {
 }
 
 // ---------------------------------------------------------------------------
-// topology: refined face t = (v, ep(v->next), nv + f, ep(prev->v))
+// topology: refined face t = (v, ep(v->next), nv + f, ep(prev->v)) in the
+// mesh's local numbering, plus the segment's id offset
 
-__global__ void k_topology_quad(const int32_t* __restrict__ ids, const int32_t* __restrict__ eout, int32_t nv,
-                                int32_t nf, int32_t* __restrict__ rids, int32_t* __restrict__ rptr,
-                                int32_t* __restrict__ rvals) {
+__global__ void k_topology_quad(const int32_t* __restrict__ ids, const int32_t* __restrict__ eout, int32_t nf,
+                                SegTable seg) {
   int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
+  const SegDesc* S = seg.d + seg_of(seg.fbase, seg.n, f);
+  const int32_t lf = (int32_t)(f - S->fb);
+  const int32_t vb = (int32_t)S->vb;
+  const int32_t off = (int32_t)S->id_offset;
+  const int32_t eb = S->nv + S->nf + off;
+  const int32_t fpid = S->nv + lf + off;
   int4 c = reinterpret_cast<const int4*>(ids)[f];
   int4 e = reinterpret_cast<const int4*>(eout)[f];
-  const int32_t eb = nv + nf;
-  const int32_t fpid = nv + f;
-  int4* o = reinterpret_cast<int4*>(rids) + 4 * (int64_t)f;
+  c.x += off - vb;
+  c.y += off - vb;
+  c.z += off - vb;
+  c.w += off - vb;
+  int4* o = reinterpret_cast<int4*>(S->dst_ids) + 4 * (int64_t)lf;
   o[0] = make_int4(c.x, eb + e.x, fpid, eb + e.w);
   o[1] = make_int4(c.y, eb + e.y, fpid, eb + e.x);
   o[2] = make_int4(c.z, eb + e.z, fpid, eb + e.y);
   o[3] = make_int4(c.w, eb + e.w, fpid, eb + e.z);
-  if (rptr) {  // createQuadColPtr: ptr[t] = 4t
-    int32_t b = 16 * f;
-    reinterpret_cast<int4*>(rptr)[f] = make_int4(b, b + 4, b + 8, b + 12);
-    if (f == nf - 1) rptr[4 * (int64_t)nf] = 16 * nf;
+  if (S->dst_ptr) {  // createQuadColPtr: ptr[t] = 4t
+    int32_t b = 16 * lf;
+    reinterpret_cast<int4*>(S->dst_ptr)[lf] = make_int4(b, b + 4, b + 8, b + 12);
+    if (lf == S->nf - 1) S->dst_ptr[4 * (int64_t)S->nf] = 16 * S->nf;
   }
-  if (rvals) {  // createQuadVals: vals[i] = i % 4 + 1
-    int4* ov = reinterpret_cast<int4*>(rvals) + 4 * (int64_t)f;
+  if (S->dst_vals) {  // createQuadVals: vals[i] = i % 4 + 1
+    int4* ov = reinterpret_cast<int4*>(S->dst_vals) + 4 * (int64_t)lf;
     int4 one = make_int4(1, 2, 3, 4);
     ov[0] = one;
     ov[1] = one;
@@ -244,20 +290,26 @@ __global__ void k_topology_quad(const int32_t* __restrict__ ids, const int32_t* 
 }
 
 __global__ void k_topology_poly(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr,
-                                const int32_t* __restrict__ eout, int32_t nv, int32_t nf, int64_t nnz,
-                                int32_t* __restrict__ rids, int32_t* __restrict__ rptr, int32_t* __restrict__ rvals) {
+                                const int32_t* __restrict__ eout, int32_t nf, SegTable seg) {
   int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
-  const int32_t eb = nv + nf;
+  const SegDesc* S = seg.d + seg_of(seg.fbase, seg.n, f);
+  const int32_t lf = (int32_t)(f - S->fb);
+  const int32_t vb = (int32_t)S->vb;
+  const int32_t off = (int32_t)S->id_offset;
+  const int32_t eb = S->nv + S->nf + off;
+  const int32_t fpid = S->nv + lf + off;
+  const int64_t cb = S->cb;
   int32_t s = fptr[f], e = fptr[f + 1];
-  int4* o = reinterpret_cast<int4*>(rids);
+  int4* o = reinterpret_cast<int4*>(S->dst_ids);
   for (int32_t t = s; t < e; ++t) {
     int32_t tp = (t == s) ? e - 1 : t - 1;
-    o[t] = make_int4(ids[t], eb + eout[t], nv + f, eb + eout[tp]);
-    if (rptr) rptr[t] = 4 * t;
-    if (rvals) reinterpret_cast<int4*>(rvals)[t] = make_int4(1, 2, 3, 4);
+    int64_t lt = t - cb;
+    o[lt] = make_int4(ids[t] - vb + off, eb + eout[t], fpid, eb + eout[tp]);
+    if (S->dst_ptr) S->dst_ptr[lt] = (int32_t)(4 * lt);
+    if (S->dst_vals) reinterpret_cast<int4*>(S->dst_vals)[lt] = make_int4(1, 2, 3, 4);
   }
-  if (rptr && f == nf - 1) rptr[nnz] = (int32_t)(4 * nnz);
+  if (S->dst_ptr && lf == S->nf - 1) S->dst_ptr[S->nnz] = (int32_t)(4 * S->nnz);
 }
 
 // ---------------------------------------------------------------------------
@@ -351,19 +403,26 @@ void launch_fill_i32(int32_t* p, int64_t n, int32_t value, hipStream_t st) {
   k_fill_i32<<<(unsigned)g, kBlock, 0, st>>>(p, n, value);
 }
 
-void launch_validate(const int32_t* ptr, const int32_t* ids, const int32_t* vals, int32_t stride, int32_t nf,
-                     int32_t nv, int64_t nnz, bool require_quad, int32_t* ids_pos, int32_t* cface, int* flags,
-                     hipStream_t st) {
-  if (nf <= 0) return;
-  k_validate<<<grid_for(nf), kBlock, 0, st>>>(ptr, ids, vals, stride, nf, nv, nnz, require_quad ? 1 : 0, ids_pos,
-                                              cface, flags);
+void launch_gather(const SegTable& seg, int32_t nf_total, bool use_vals, int32_t* ids_pos, int32_t* fptr,
+                   int32_t* cface, int* flags, hipStream_t st) {
+  if (nf_total <= 0) return;
+  k_gather<<<grid_for(nf_total), kBlock, 0, st>>>(seg, nf_total, use_vals ? 1 : 0, ids_pos, fptr, cface, flags);
+}
+
+void launch_gather_verts(const SegTable& seg, int32_t nv_total, float4* P, hipStream_t st) {
+  if (nv_total <= 0) return;
+  k_gather_verts<<<grid_for(nv_total), kBlock, 0, st>>>(seg, nv_total, P);
+}
+
+void launch_seg_bounds(const int64_t* vbase, int32_t n, const int32_t* eptr, int32_t* out, hipStream_t st) {
+  k_seg_bounds<<<grid_for(n + 1), kBlock, 0, st>>>(vbase, n, eptr, out);
 }
 
 void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int32_t* fptr, const float4* P, int32_t nf,
-                  int32_t nv, float4* out, int32_t* cnt, hipStream_t st) {
+                  const SegTable& seg, int32_t* cnt, hipStream_t st) {
   if (nf <= 0) return;
   unsigned g = grid_for(nf);
-#define SLAK_FACES(Q, A, B) k_faces<Q, A, B><<<g, kBlock, 0, st>>>(ids, fptr, P, nf, nv, out, cnt)
+#define SLAK_FACES(Q, A, B) k_faces<Q, A, B><<<g, kBlock, 0, st>>>(ids, fptr, P, nf, seg, cnt)
   if (quad) {
     if (fp && count) SLAK_FACES(true, true, true);
     else if (fp) SLAK_FACES(true, true, false);
@@ -417,13 +476,13 @@ void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st) {
     k_spgemm_fde<false><<<grid_for(L.nv), kBlock, 0, st>>>(L);
 }
 
-void launch_topology(bool quad, const int32_t* ids, const int32_t* fptr, const int32_t* eout, int32_t nv, int32_t nf,
-                     int64_t nnz, int32_t* rids, int32_t* rptr, int32_t* rvals, hipStream_t st) {
+void launch_topology(bool quad, const int32_t* ids, const int32_t* fptr, const int32_t* eout, int32_t nf,
+                     const SegTable& seg, hipStream_t st) {
   if (nf <= 0) return;
   if (quad)
-    k_topology_quad<<<grid_for(nf), kBlock, 0, st>>>(ids, eout, nv, nf, rids, rptr, rvals);
+    k_topology_quad<<<grid_for(nf), kBlock, 0, st>>>(ids, eout, nf, seg);
   else
-    k_topology_poly<<<grid_for(nf), kBlock, 0, st>>>(ids, fptr, eout, nv, nf, nnz, rids, rptr, rvals);
+    k_topology_poly<<<grid_for(nf), kBlock, 0, st>>>(ids, fptr, eout, nf, seg);
 }
 
 int64_t scan_partials_len(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }

@@ -1,0 +1,566 @@
+// Level plans over a union of meshes (see slak_plan.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "slak_plan.hpp"
+
+namespace slak {
+
+std::string flags_message(int f) {
+  std::string m;
+  if (f & kFlagBadPtr) m += " ptr is not a valid CSR offset array (ptr[0]=0, non-decreasing, ptr[nf]=nnz);";
+  if (f & kFlagBadOrder) m += " face with fewer than three vertices;";
+  if (f & kFlagBadId) m += " vertex id out of range;";
+  if (f & kFlagBadVals) m += " vals is not a permutation of 1..face order;";
+  if (f & kFlagBadQuad) m += " QUAD mesh with a face that is not a quad at ptr = 4i;";
+  if (f & kFlagDupHalfEdge)
+    m += " non-manifold: a directed edge occurs twice (inconsistent orientation or >2 faces on an edge);";
+  if (f & kFlagDegenerate) m += " degenerate face (repeated vertex along a face);";
+  if (f & kFlagOverflow) m += " index overflow (counts exceed int32);";
+  return m;
+}
+
+namespace {
+
+int64_t* pinned_buffer(size_t n) {
+  struct Pinned {
+    int64_t* p = nullptr;
+    size_t n = 0;
+    ~Pinned() {
+      if (p) (void)hipHostFree(p);
+    }
+  };
+  thread_local Pinned pin;
+  if (pin.n < n) {
+    if (pin.p) (void)hipHostFree(pin.p);
+    pin.p = nullptr;
+    pin.n = 0;
+    SLAK_HIP_CHECK(hipHostMalloc((void**)&pin.p, n * sizeof(int64_t)));
+    pin.n = n;
+  }
+  return pin.p;
+}
+
+void check_i32(int64_t x, const std::string& what) {
+  if (x < 0 || x > 0x7fffffffll)
+    throw Error(SLAK_ERR_INVALID_ARGUMENT, "too large for int32 indices: " + what + " = " + std::to_string(x));
+}
+
+}  // namespace
+
+Plan::Plan(int scheme, std::vector<MeshJob> jobs, OutputMode mode, hipStream_t st)
+    : scheme_(scheme), jobs_(std::move(jobs)), mode_(mode), st_(st), arena_(pool()) {}
+
+Plan::~Plan() {
+  if (exec_) (void)hipGraphExecDestroy(exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  bool held = false;
+  for (auto& j : jobs_) held |= j.out_ids != nullptr;
+  if (!outputs_released_ && held) {
+    (void)hipStreamSynchronize(st_);
+    if (mode_ == OutputMode::kSeparate)
+      for (auto& j : jobs_)
+        for (void* p : {(void*)j.out_verts, (void*)j.out_ids, (void*)j.out_ptr, (void*)j.out_vals})
+          if (p) pool().release(p);
+  }
+  if (!arena_.empty()) (void)hipStreamSynchronize(st_);
+}
+
+int Plan::readback(const int32_t* src, int count, int64_t* dst) {
+  int64_t* pin = pinned_buffer((size_t)count / 2 + 2);
+  int32_t* p32 = reinterpret_cast<int32_t*>(pin);
+  SLAK_HIP_CHECK(hipMemcpyAsync(p32, d_flags_, sizeof(int), hipMemcpyDeviceToHost, st_));
+  if (count > 0) SLAK_HIP_CHECK(hipMemcpyAsync(p32 + 1, src, count * sizeof(int32_t), hipMemcpyDeviceToHost, st_));
+  SLAK_HIP_CHECK(hipStreamSynchronize(st_));
+  ++syncs_;
+  for (int i = 0; i < count; ++i) dst[i] = p32[1 + i];
+  return p32[0];
+}
+
+void Plan::upload_tables(LevelPlan& L) {
+  size_t n = L.segs.size();
+  if (!L.d_segs) {
+    L.d_segs = arena_.get_n<SegDesc>((int64_t)n);
+    L.d_vbase = arena_.get_n<int64_t>((int64_t)n + 1);
+    L.d_fbase = arena_.get_n<int64_t>((int64_t)n + 1);
+    L.d_mflags = arena_.get_n<int>((int64_t)n);
+  }
+  SLAK_HIP_CHECK(hipMemcpyAsync(L.d_segs, L.segs.data(), n * sizeof(SegDesc), hipMemcpyHostToDevice, st_));
+  SLAK_HIP_CHECK(hipMemcpyAsync(L.d_vbase, L.vbase.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st_));
+  SLAK_HIP_CHECK(hipMemcpyAsync(L.d_fbase, L.fbase.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st_));
+  // the host vectors live as long as the plan, past every copy's completion
+}
+
+std::string Plan::describe_failure(int fl) const {
+  std::string m;
+  for (size_t j = 0; j < jobs_.size(); ++j)
+    if (jobs_[j].bad_flags) {
+      m = "mesh " + std::to_string(j) + ":" + flags_message(jobs_[j].bad_flags);
+      break;
+    }
+  if (m.empty()) m = flags_message(fl);
+  return "invalid mesh: " + m;
+}
+
+// First level: per-job bases, the caller meshes gathered into the union.
+void Plan::plan_first_level() {
+  levels_.clear();
+  levels_.emplace_back();
+  LevelPlan& L = levels_[0];
+  bool all_quad = true;
+  for (size_t j = 0; j < jobs_.size(); ++j) {
+    MeshJob& J = jobs_[j];
+    if (J.levels <= 0) continue;
+    all_quad &= J.in.type == SLAK_QUAD;
+  }
+  L.quad = all_quad;
+  int64_t vb = 0, fb = 0, cb = 0;
+  for (size_t j = 0; j < jobs_.size(); ++j) {
+    MeshJob& J = jobs_[j];
+    if (J.levels <= 0) continue;
+    SegDesc S{};
+    S.vb = vb;
+    S.fb = fb;
+    S.cb = cb;
+    S.nv = (int32_t)J.in.nverts;
+    S.nf = (int32_t)J.in.nfaces;
+    S.nnz = (int64_t)J.in.nnz;
+    S.src_ptr = J.in.ptr;
+    S.src_ids = J.in.ids;
+    S.src_vals = J.in.vals;
+    S.src_verts = reinterpret_cast<const float4*>(J.in.verts);
+    S.src_stride = J.in.ptr ? 0 : (J.in.type == SLAK_QUAD ? 4 : 3);
+    S.require_quad = J.in.type == SLAK_QUAD;
+    L.jobs.push_back((int)j);
+    L.segs.push_back(S);
+    L.vbase.push_back(vb);
+    L.fbase.push_back(fb);
+    vb += S.nv;
+    fb += S.nf;
+    cb += S.nnz;
+  }
+  L.vbase.push_back(vb);
+  L.fbase.push_back(fb);
+  L.nv = vb;
+  L.nf = fb;
+  L.nnz = cb;
+  check_i32(L.nv, "vertices of the first level");
+  check_i32(L.nf, "faces of the first level");
+  check_i32(L.nnz, "corners of the first level");
+  check_i32(4 * L.nnz, "refined corners of the first level");
+
+  int32_t* ids_pos = arena_.get_n<int32_t>(L.nnz);
+  L.ids = ids_pos;
+  if (!L.quad) {
+    L.fptr = arena_.get_n<int32_t>(L.nf + 1);
+    L.cface = arena_.get_n<int32_t>(L.nnz);
+  }
+  if (L.segs.size() == 1) {
+    L.P = L.segs[0].src_verts;
+  } else {
+    P1_ = arena_.get_n<float4>(L.nv);
+    L.P = P1_;
+  }
+  L.cnt = arena_.get_n<int32_t>(L.nv);
+  L.tptr = arena_.get_n<int32_t>(L.nv + 1);
+  L.tids = arena_.get_n<int32_t>(L.nnz);
+  L.ecnt = arena_.get_n<int32_t>(L.nv);
+  L.eptr = arena_.get_n<int32_t>(L.nv + 1);
+  L.eout = arena_.get_n<int32_t>(L.nnz);
+}
+
+void Plan::record_analysis(LevelPlan& L, bool count) {
+  const double poly_b = L.quad ? 0.0 : 4.0 * (L.nf + 1) + 4.0 * L.nnz;
+  if (count) {
+    {
+      ProfScope ps(st_, "fill", 4.0 * L.nv);
+      launch_fill_i32(L.cnt, L.nv, 0, st_);
+    }
+    ProfScope ps(st_, L.quad ? "count_quad" : "count_poly", 4.0 * L.nnz + 8.0 * L.nv + (L.quad ? 0.0 : 4.0 * L.nf));
+    launch_faces(L.quad, false, true, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), L.cnt, st_);
+  }
+  {
+    ProfScope ps(st_, "scan", 8.0 * L.nv + 4);
+    launch_scan(L.cnt, L.tptr, L.nv, d_partial_, d_flags_, st_);
+  }
+  {
+    ProfScope ps(st_, L.quad ? "transpose_quad" : "transpose_poly",
+                 8.0 * L.nnz + 12.0 * L.nv + (L.quad ? 0.0 : 4.0 * L.nf));
+    launch_transpose(L.quad, L.ids, L.fptr, (int32_t)L.nf, L.tptr, L.cnt, L.tids, st_);
+  }
+  LevelView V{};
+  V.ids = L.ids;
+  V.fptr = L.fptr;
+  V.cface = L.cface;
+  V.P = L.P;
+  V.nv = (int32_t)L.nv;
+  V.nf = (int32_t)L.nf;
+  V.nnz = L.nnz;
+  V.tptr = L.tptr;
+  V.tids = L.tids;
+  V.ecnt = L.ecnt;
+  V.flags = d_flags_;
+  V.seg = L.table();
+  {
+    ProfScope ps(st_, L.quad ? "spgemm_symbolic_quad" : "spgemm_symbolic_poly", 8.0 * L.nnz + 8.0 * L.nv + poly_b);
+    launch_symbolic(L.quad, scheme_, V, st_);
+  }
+  {
+    ProfScope ps(st_, "scan", 8.0 * L.nv + 4);
+    launch_scan(L.ecnt, L.eptr, L.nv, d_partial_, d_flags_, st_);
+  }
+}
+
+void Plan::prepare() {
+  for (auto& J : jobs_) {
+    if (J.in.type != SLAK_TRI && J.in.type != SLAK_QUAD && J.in.type != SLAK_POLY)
+      throw Error(SLAK_ERR_INVALID_ARGUMENT, "unknown mesh type");
+    if (J.target < 0 || J.target > 12) throw Error(SLAK_ERR_INVALID_ARGUMENT, "target level out of range");
+    J.levels = J.in.type != SLAK_QUAD ? std::max(1, J.target) : J.target;
+    if (J.levels <= 0) continue;
+    if (!J.in.ids || !J.in.verts) throw Error(SLAK_ERR_INVALID_ARGUMENT, "mesh ids/verts must not be null");
+    if (!J.in.ptr) {
+      if (J.in.type == SLAK_POLY) throw Error(SLAK_ERR_INVALID_ARGUMENT, "POLY mesh without ptr");
+      int k = J.in.type == SLAK_QUAD ? 4 : 3;
+      if ((int64_t)J.in.nnz != k * (int64_t)J.in.nfaces)
+        throw Error(SLAK_ERR_INVALID_ARGUMENT, "nnz != face order * nfaces");
+    }
+    check_i32((int64_t)J.in.nverts, "nverts");
+    check_i32((int64_t)J.in.nfaces, "nfaces");
+    check_i32((int64_t)J.in.nnz, "nnz");
+  }
+  d_flags_ = arena_.get_n<int>(1);
+  bool any = false;
+  for (auto& J : jobs_) any |= J.levels > 0;
+  if (!any) {
+    prepared_ = true;
+    return;
+  }
+  plan_first_level();
+  LevelPlan& L1 = levels_[0];
+  const int nseg = (int)L1.segs.size();
+  upload_tables(L1);
+  int64_t need = scan_partials_len(L1.nv);
+  d_partial_ = arena_.get_n<int64_t>(need);
+  partial_len_ = need;
+
+  // 1. validate + gather
+  launch_fill_i32(d_flags_, 1, 0, st_);
+  launch_fill_i32(L1.d_mflags, nseg, 0, st_);
+  {
+    ProfScope ps(st_, "gather", 4.0 * (L1.nf + 1) + 12.0 * L1.nnz + (L1.quad ? 0.0 : 8.0 * L1.nnz + 4.0 * L1.nf));
+    launch_gather(L1.table(), (int32_t)L1.nf, scheme_ == kLAK, const_cast<int32_t*>(L1.ids),
+                  const_cast<int32_t*>(L1.fptr), const_cast<int32_t*>(L1.cface), d_flags_, st_);
+  }
+  if (P1_) launch_gather_verts(L1.table(), (int32_t)L1.nv, P1_, st_);
+  auto fail_if = [&](int fl) {
+    if (!fl) return;
+    std::vector<int> mf(nseg, 0);
+    SLAK_HIP_CHECK(hipMemcpy(mf.data(), L1.d_mflags, nseg * sizeof(int), hipMemcpyDeviceToHost));
+    for (int k = 0; k < nseg; ++k) jobs_[L1.jobs[k]].bad_flags = mf[k];
+    throw Error(SLAK_ERR_INVALID_MESH, describe_failure(fl));
+  };
+  fail_if(readback(nullptr, 0, nullptr));
+
+  // 2. first-level analysis: M^T, owned-edge counts, per-mesh edge totals
+  record_analysis(L1, true);
+  d_bounds_ = arena_.get_n<int32_t>(nseg + 1);
+  launch_seg_bounds(L1.d_vbase, nseg, L1.eptr, d_bounds_, st_);
+  std::vector<int64_t> bounds(nseg + 1);
+  fail_if(readback(d_bounds_, nseg + 1, bounds.data()));
+
+  // 3. closed-form sizes of every later level
+  int Lmax = 0;
+  for (int k = 0; k < nseg; ++k) {
+    MeshJob& J = jobs_[L1.jobs[k]];
+    int lv = J.levels;
+    Lmax = std::max(Lmax, lv);
+    J.nv.assign(lv + 1, 0);
+    J.nf.assign(lv + 1, 0);
+    J.nnz.assign(lv + 1, 0);
+    J.ne.assign(lv + 1, 0);
+    J.nv[0] = (int64_t)J.in.nverts;
+    J.nf[0] = (int64_t)J.in.nfaces;
+    J.nnz[0] = (int64_t)J.in.nnz;
+    J.ne[0] = bounds[k + 1] - bounds[k];
+    int64_t next = 2 * J.ne[0] - J.nnz[0];  // boundary edges
+    if (next < 0) throw Error(SLAK_ERR_INTERNAL, "inconsistent edge count");
+    for (int l = 1; l <= lv; ++l) {
+      J.nv[l] = J.nv[l - 1] + J.nf[l - 1] + J.ne[l - 1];
+      J.nf[l] = J.nnz[l - 1];
+      J.nnz[l] = 4 * J.nnz[l - 1];
+      int64_t nx = next << l;
+      J.ne[l] = (J.nnz[l] - nx) / 2 + nx;
+      check_i32(J.nv[l], "refined vertices");
+      check_i32(J.nnz[l], "refined corners");
+    }
+  }
+
+  // 4. level unions, destinations and outputs
+  const bool lak_form = scheme_ == kLAK;
+  auto out_alloc = [&](size_t bytes) -> void* {
+    if (mode_ == OutputMode::kSeparate) return pool().alloc(bytes);
+    return nullptr;  // carved from pooled blocks below
+  };
+  // pooled outputs: one block per kind
+  int64_t tot_v = 0, tot_i = 0, tot_p = 0;
+  for (auto& J : jobs_)
+    if (J.levels > 0) {
+      tot_v += J.nv[J.levels];
+      tot_i += 4 * J.nnz[J.levels - 1];
+      tot_p += J.nf[J.levels] + 1;
+    }
+  float4* pool_v = nullptr;
+  int32_t *pool_i = nullptr, *pool_p = nullptr, *pool_a = nullptr;
+  if (mode_ == OutputMode::kPooled) {
+    pool_v = arena_.get_n<float4>(tot_v);
+    pool_i = arena_.get_n<int32_t>(tot_i);
+    if (lak_form) {
+      pool_p = arena_.get_n<int32_t>(tot_p);
+      pool_a = arena_.get_n<int32_t>(tot_i);
+    }
+  }
+  int64_t ov = 0, oi = 0, op = 0;
+  for (auto& J : jobs_) {
+    if (J.levels <= 0) continue;
+    int64_t nvo = J.nv[J.levels], nio = 4 * J.nnz[J.levels - 1], npo = J.nf[J.levels] + 1;
+    if (mode_ == OutputMode::kSeparate) {
+      J.out_verts = static_cast<float4*>(out_alloc(nvo * sizeof(float4)));
+      J.out_ids = static_cast<int32_t*>(out_alloc(nio * sizeof(int32_t)));
+      if (lak_form) {
+        J.out_ptr = static_cast<int32_t*>(out_alloc(npo * sizeof(int32_t)));
+        J.out_vals = static_cast<int32_t*>(out_alloc(nio * sizeof(int32_t)));
+      }
+    } else {
+      J.out_verts = pool_v + ov;
+      J.out_ids = pool_i + oi;
+      if (lak_form) {
+        J.out_ptr = pool_p + op;
+        J.out_vals = pool_a + oi;
+      }
+    }
+    ov += nvo;
+    oi += nio;
+    op += npo;
+  }
+
+  // level l (1-based) consumes sizes index l-1 and produces index l
+  int64_t max_nv = 0, max_nnz = 0;
+  levels_.resize(Lmax);
+  for (int l = 1; l <= Lmax; ++l) {
+    LevelPlan& L = levels_[l - 1];
+    if (l > 1) {
+      LevelPlan& prev = levels_[l - 2];
+      L.quad = true;
+      L.jobs.clear();
+      L.segs.clear();
+      L.vbase.clear();
+      L.fbase.clear();
+      int64_t vb = 0, fb = 0, cb = 0;
+      for (int j : prev.jobs) {
+        MeshJob& J = jobs_[j];
+        if (J.levels < l) continue;
+        SegDesc S{};
+        S.vb = vb;
+        S.fb = fb;
+        S.cb = cb;
+        S.nv = (int32_t)J.nv[l - 1];
+        S.nf = (int32_t)J.nf[l - 1];
+        S.nnz = J.nnz[l - 1];
+        L.jobs.push_back(j);
+        L.segs.push_back(S);
+        L.vbase.push_back(vb);
+        L.fbase.push_back(fb);
+        vb += S.nv;
+        fb += S.nf;
+        cb += S.nnz;
+      }
+      L.vbase.push_back(vb);
+      L.fbase.push_back(fb);
+      L.nv = vb;
+      L.nf = fb;
+      L.nnz = cb;
+      check_i32(L.nv, "union vertices of level " + std::to_string(l));
+      check_i32(L.nnz, "union corners of level " + std::to_string(l));
+      max_nv = std::max(max_nv, L.nv);
+      max_nnz = std::max(max_nnz, L.nnz);
+    }
+    // destinations of this level's segments
+    int64_t nv_next = 0, nnz_next = 0;
+    for (int j : L.jobs)
+      if (jobs_[j].levels > l) {
+        nv_next += jobs_[j].nv[l];
+        nnz_next += jobs_[j].nnz[l];
+      }
+    float4* Un_P = nv_next ? arena_.get_n<float4>(nv_next) : nullptr;
+    int32_t* Un_ids = nnz_next ? arena_.get_n<int32_t>(nnz_next) : nullptr;
+    int64_t vbn = 0, cbn = 0;
+    for (size_t k = 0; k < L.segs.size(); ++k) {
+      MeshJob& J = jobs_[L.jobs[k]];
+      SegDesc& S = L.segs[k];
+      S.ne = J.ne[l - 1];
+      if (J.levels == l) {
+        S.dst_verts = J.out_verts;
+        S.dst_ids = J.out_ids;
+        S.dst_ptr = J.out_ptr;
+        S.dst_vals = J.out_vals;
+        S.id_offset = 0;
+      } else {
+        S.dst_verts = Un_P + vbn;
+        S.dst_ids = Un_ids + cbn;
+        S.dst_ptr = nullptr;
+        S.dst_vals = nullptr;
+        S.id_offset = vbn;
+        vbn += J.nv[l];
+        cbn += J.nnz[l];
+      }
+    }
+    if (l < Lmax) {
+      LevelPlan& nx = levels_[l];
+      nx.ids = Un_ids;
+      nx.P = Un_P;
+    }
+  }
+  // scratch shared by levels >= 2
+  if (Lmax >= 2) {
+    int32_t* cnt = arena_.get_n<int32_t>(max_nv);
+    int32_t* tptr = arena_.get_n<int32_t>(max_nv + 1);
+    int32_t* tids = arena_.get_n<int32_t>(max_nnz);
+    int32_t* ecnt = arena_.get_n<int32_t>(max_nv);
+    int32_t* eptr = arena_.get_n<int32_t>(max_nv + 1);
+    int32_t* eout = arena_.get_n<int32_t>(max_nnz);
+    int64_t need2 = scan_partials_len(max_nv);
+    if (need2 > partial_len_) {
+      d_partial_ = arena_.get_n<int64_t>(need2);
+      partial_len_ = need2;
+    }
+    for (int l = 2; l <= Lmax; ++l) {
+      LevelPlan& L = levels_[l - 1];
+      L.cnt = cnt;
+      L.tptr = tptr;
+      L.tids = tids;
+      L.ecnt = ecnt;
+      L.eptr = eptr;
+      L.eout = eout;
+    }
+  }
+  for (auto& L : levels_) upload_tables(L);
+  prepared_ = true;
+}
+
+void Plan::record_level(int l) {
+  LevelPlan& L = levels_[l - 1];
+  const double poly_b = L.quad ? 0.0 : 4.0 * (L.nf + 1) + 4.0 * L.nnz;
+  double ne_tot = 0;
+  for (auto& S : L.segs) ne_tot += (double)S.ne;
+  if (l > 1) {
+    {
+      ProfScope ps(st_, "fill", 4.0 * L.nv);
+      launch_fill_i32(L.cnt, L.nv, 0, st_);
+    }
+    {
+      ProfScope ps(st_, "facepoints+count", 4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 8.0 * L.nv);
+      launch_faces(true, true, true, L.ids, nullptr, L.P, (int32_t)L.nf, L.table(), L.cnt, st_);
+    }
+    record_analysis(L, false);
+  } else {
+    ProfScope ps(st_, L.quad ? "facepoints_quad" : "facepoints_poly",
+                 4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + (L.quad ? 0.0 : 4.0 * (L.nf + 1)));
+    launch_faces(L.quad, true, false, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), nullptr, st_);
+  }
+  {
+    // every corner's edge id is written by the numeric pass for a valid mesh;
+    // zero-filling keeps refined ids in range even if one were not
+    ProfScope ps(st_, "fill", 4.0 * L.nnz);
+    launch_fill_i32(L.eout, L.nnz, 0, st_);
+  }
+  LevelView V{};
+  V.ids = L.ids;
+  V.fptr = L.fptr;
+  V.cface = L.cface;
+  V.P = L.P;
+  V.nv = (int32_t)L.nv;
+  V.nf = (int32_t)L.nf;
+  V.nnz = L.nnz;
+  V.tptr = L.tptr;
+  V.tids = L.tids;
+  V.eptr = L.eptr;
+  V.eout = L.eout;
+  V.flags = d_flags_;
+  V.seg = L.table();
+  {
+    // reads tptr, tids, ids, eptr, P, face points; writes vertex points, edge points, eout
+    ProfScope ps(st_, L.quad ? "spgemm_numeric_quad" : "spgemm_numeric_poly",
+                 8.0 * L.nv + 8.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 16.0 * L.nv + 16.0 * ne_tot + 4.0 * L.nnz +
+                     4.0 * L.nv + poly_b);
+    launch_numeric(L.quad, scheme_, V, st_);
+  }
+  {
+    bool lak_final = false;
+    double nnz_final = 0;
+    for (size_t k = 0; k < L.segs.size(); ++k)
+      if (L.segs[k].dst_ptr) {
+        lak_final = true;
+        nnz_final += (double)L.segs[k].nnz;
+      }
+    ProfScope ps(st_, L.quad ? "topology_quad" : "topology_poly",
+                 8.0 * L.nnz + 16.0 * L.nnz + (lak_final ? 20.0 * nnz_final : 0.0) + (L.quad ? 0.0 : 4.0 * L.nf));
+    launch_topology(L.quad, L.ids, L.fptr, L.eout, (int32_t)L.nf, L.table(), st_);
+  }
+}
+
+void Plan::launch(bool include_analysis) {
+  if (levels_.empty()) return;
+  LevelPlan& L1 = levels_[0];
+  if (include_analysis) {
+    launch_fill_i32(d_flags_, 1, 0, st_);
+    launch_fill_i32(L1.d_mflags, (int64_t)L1.segs.size(), 0, st_);
+    {
+      ProfScope ps(st_, "gather", 4.0 * (L1.nf + 1) + 12.0 * L1.nnz + (L1.quad ? 0.0 : 8.0 * L1.nnz + 4.0 * L1.nf));
+      launch_gather(L1.table(), (int32_t)L1.nf, scheme_ == kLAK, const_cast<int32_t*>(L1.ids),
+                    const_cast<int32_t*>(L1.fptr), const_cast<int32_t*>(L1.cface), d_flags_, st_);
+    }
+    if (P1_) {
+      ProfScope ps(st_, "gather_verts", 32.0 * L1.nv);
+      launch_gather_verts(L1.table(), (int32_t)L1.nv, P1_, st_);
+    }
+    record_analysis(L1, true);
+  }
+  for (int l = 1; l <= (int)levels_.size(); ++l) record_level(l);
+}
+
+void Plan::run_graph() {
+  if (levels_.empty()) return;
+  if (!exec_) {
+    // event-pair timing stays out of the captured graph
+    const bool prof = Profiler::get().enabled();
+    Profiler::get().set_enabled(false);
+    SLAK_HIP_CHECK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+    try {
+      launch(true);
+      Profiler::get().set_enabled(prof);
+    } catch (...) {
+      Profiler::get().set_enabled(prof);
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(st_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    SLAK_HIP_CHECK(hipStreamEndCapture(st_, &graph_));
+    SLAK_HIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+  }
+  SLAK_HIP_CHECK(hipGraphLaunch(exec_, st_));
+}
+
+void Plan::finish() {
+  int fl = readback(nullptr, 0, nullptr);
+  if (fl) throw Error(SLAK_ERR_INTERNAL, "refinement produced invalid topology:" + flags_message(fl));
+}
+
+void Plan::release_outputs() { outputs_released_ = true; }
+
+size_t Plan::bytes() const { return arena_.bytes(); }
+
+}  // namespace slak

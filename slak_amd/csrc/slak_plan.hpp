@@ -1,0 +1,117 @@
+// Multi-mesh level plans: the level chain of subdivideIteratively
+// (LAKCatmullClarkImpl.cu:512-571, CatmullClarkImpl.cu:457-515) over a union
+// of meshes, with every size after the first level's edge count known in
+// closed form, so the whole chain can be launched back to back or captured in
+// one hipGraph.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "slak_c.h"
+#include "slak_internal.hpp"
+
+namespace slak {
+
+// One caller mesh of a plan.
+struct MeshJob {
+  slak_mesh in{};   // caller's device mesh (never modified)
+  int target = 0;   // requested level
+  int levels = 0;   // levels to refine (a non-quad input gets at least one)
+  // closed-form sizes of the input of level l (l = 0..levels) and its edges
+  std::vector<int64_t> nv, nf, nnz, ne;
+  // final outputs
+  float4* out_verts = nullptr;
+  int32_t* out_ids = nullptr;
+  int32_t* out_ptr = nullptr;
+  int32_t* out_vals = nullptr;
+  int bad_flags = 0;
+};
+
+struct LevelPlan {
+  bool quad = true;
+  int64_t nv = 0, nf = 0, nnz = 0;  // union sizes
+  const int32_t* ids = nullptr;
+  const int32_t* fptr = nullptr;
+  const int32_t* cface = nullptr;
+  const float4* P = nullptr;
+  // scratch (the first level owns its own; later levels share the largest)
+  int32_t *cnt = nullptr, *tptr = nullptr, *tids = nullptr, *ecnt = nullptr, *eptr = nullptr, *eout = nullptr;
+  std::vector<int> jobs;  // segment -> job
+  std::vector<SegDesc> segs;
+  std::vector<int64_t> vbase, fbase;
+  SegDesc* d_segs = nullptr;
+  int64_t* d_vbase = nullptr;
+  int64_t* d_fbase = nullptr;
+  int* d_mflags = nullptr;
+  SegTable table() const { return SegTable{d_segs, d_vbase, d_fbase, (int32_t)segs.size(), d_mflags}; }
+};
+
+enum class OutputMode {
+  kSeparate,  // one pool block per output array (handed to the caller)
+  kPooled,    // one block per array kind for all meshes (owned by the plan)
+};
+
+class Plan {
+ public:
+  Plan(int scheme, std::vector<MeshJob> jobs, OutputMode mode, hipStream_t st);
+  ~Plan();
+  Plan(const Plan&) = delete;
+  Plan& operator=(const Plan&) = delete;
+
+  // Validates the inputs and runs the first level's analysis (two blocking
+  // round trips), then sizes and allocates everything. Throws on bad input.
+  void prepare();
+  // Launches the chain on the plan's stream. With include_analysis the first
+  // level is recomputed from the caller's meshes (used for graph replays).
+  void launch(bool include_analysis);
+  // Captures launch(true) into a hipGraph (once) and replays it.
+  void run_graph();
+  // Waits for the stream and checks the device flag word.
+  void finish();
+  // Hands the outputs of kSeparate plans to the caller (not freed by the plan).
+  void release_outputs();
+
+  std::vector<MeshJob>& jobs() { return jobs_; }
+  LevelPlan& level(int i) { return levels_[i]; }
+  int* flags() { return d_flags_; }
+  Arena& arena() { return arena_; }
+  int syncs() const { return syncs_; }
+  hipStream_t stream() const { return st_; }
+  size_t bytes() const;
+  int max_levels() const { return (int)levels_.size(); }
+  std::string describe_failure(int fl) const;
+
+ private:
+  void plan_first_level();
+  void record_analysis(LevelPlan& L, bool count);
+  void record_level(int l);
+  void upload_tables(LevelPlan& L);
+  int readback(const int32_t* src, int count, int64_t* dst);
+
+  int scheme_;
+  std::vector<MeshJob> jobs_;
+  OutputMode mode_;
+  hipStream_t st_;
+  Arena arena_;
+  std::vector<LevelPlan> levels_;
+  int* d_flags_ = nullptr;
+  int64_t* d_partial_ = nullptr;
+  int64_t partial_len_ = 0;
+  int32_t* d_bounds_ = nullptr;
+  // first-level union gather (multi-mesh)
+  float4* P1_ = nullptr;
+  bool prepared_ = false;
+  bool outputs_released_ = false;
+  int syncs_ = 0;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+  std::vector<void*> pooled_outputs_;
+};
+
+std::string flags_message(int f);
+
+}  // namespace slak

@@ -1,0 +1,143 @@
+"""Batched subdivision (slak_batch_*) against per-mesh calls and the oracle.
+
+The batched chain runs the same kernels over a union of meshes, so each
+mesh's result must equal its single-mesh result bit for bit, graph or not,
+replay after replay."""
+import numpy as np
+import pytest
+
+from slak_amd import capi, meshgen as g
+from slak_amd.batch import Batch
+from slak_amd.capi import SlakError
+from slak_amd.mesh import MeshInfo, subdivide_host, to_host
+
+pytestmark = pytest.mark.gpu
+
+LAK, SLAK = capi.SLAK_SCHEME_LAK, capi.SLAK_SCHEME_SLAK
+
+
+def fetch(m: MeshInfo):
+    ids = to_host(m.ids, m.nnz, np.int32)
+    verts = to_host(m.verts, 4 * m.nverts, np.float32).reshape(-1, 4)
+    ptr = to_host(m.ptr, m.nfaces + 1, np.int32) if m.ptr else None
+    vals = to_host(m.vals, m.nnz, np.int32) if m.vals else None
+    return ids, verts, ptr, vals
+
+
+def mixed_batch(n, seed):
+    rng = np.random.default_rng(seed)
+    meshes, levels = [], []
+    for i in range(n):
+        kind = i % 5
+        if kind == 0:
+            m = g.torus(int(rng.integers(3, 12)), int(rng.integers(3, 10)), seed=seed, mesh=i)
+        elif kind == 1:
+            m = g.torus_mixed(int(rng.integers(4, 14)), int(rng.integers(3, 9)), 0.3, pentagons=True, seed=seed, mesh=i)
+        elif kind == 2:
+            m = g.open_grid(int(rng.integers(1, 6)), int(rng.integers(1, 5)), sigma=0.01, seed=seed, mesh=i)
+        elif kind == 3:
+            m = g.cube() if i % 2 else g.tetrahedron()
+        else:
+            m = g.torus_mixed(int(rng.integers(4, 10)), int(rng.integers(4, 8)), 0.5, seed=seed, mesh=i)
+        meshes.append(m)
+        levels.append(int(rng.integers(0, 4)))
+    return meshes, levels
+
+
+@pytest.mark.parametrize("scheme", [LAK, SLAK], ids=["LAK", "SLAK"])
+def test_batch_equals_single_mesh_calls(oracle, scheme):
+    hosts, levels = mixed_batch(40, seed=3)
+    dev = [MeshInfo.from_host(h) for h in hosts]
+    try:
+        with Batch(dev, levels, scheme) as b:
+            b.run(use_graph=False)
+            for i, (h, lv) in enumerate(zip(hosts, levels)):
+                r = b.result(i)
+                if h.type == g.QUAD and lv == 0:
+                    assert r.ids == 0  # untouched, as subdivideIteratively leaves it
+                    continue
+                ids, verts, ptr, vals = fetch(r)
+                single, _ = subdivide_host(h, lv, scheme)
+                np.testing.assert_array_equal(ids, single.ids, err_msg=f"mesh {i}")
+                np.testing.assert_array_equal(verts, single.verts, err_msg=f"mesh {i}")
+                if scheme == LAK:
+                    np.testing.assert_array_equal(ptr, single.ptr)
+                    np.testing.assert_array_equal(vals, single.vals)
+                else:
+                    assert ptr is None and vals is None
+                if i % 7 == 0:
+                    ref = oracle.subdivide(h, lv, scheme=scheme)
+                    np.testing.assert_array_equal(ids, ref.ids)
+                    assert np.abs(verts - ref.P).max() / h.bbox_diag() <= 1e-5
+    finally:
+        for d in dev:
+            d.freeAndReset()
+
+
+def test_graph_replays_match_eager_and_track_new_positions():
+    hosts, levels = mixed_batch(25, seed=8)
+    levels = [max(1, x) for x in levels]
+    dev = [MeshInfo.from_host(h) for h in hosts]
+    try:
+        with Batch(dev, levels, LAK) as b:
+            b.run(use_graph=False)
+            eager = [fetch(b.result(i)) for i in range(len(hosts))]
+            for rep in range(3):
+                b.run(use_graph=True)
+                # eager work between replays
+                subdivide_host(g.cube(), 1, LAK)
+                for i in range(len(hosts)):
+                    got = fetch(b.result(i))
+                    np.testing.assert_array_equal(got[0], eager[i][0])
+                    np.testing.assert_array_equal(got[1], eager[i][1])
+            # new positions, same topology: the replay recomputes from the inputs
+            for d, h in zip(dev, hosts):
+                moved = (h.verts.copy())
+                moved[:, :3] *= 2.0
+                capi.check(capi.lib().slak_memcpy_h2d(d.verts, moved.ctypes.data, moved.nbytes))
+            b.run(use_graph=True)
+            for i, h in enumerate(hosts):
+                moved = g.HostMesh(h.type, h.ptr, h.ids, h.vals, h.verts.copy())
+                moved.verts[:, :3] *= 2.0
+                single, _ = subdivide_host(moved, levels[i], LAK)
+                got = fetch(b.result(i))
+                np.testing.assert_array_equal(got[1], single.verts)
+    finally:
+        for d in dev:
+            d.freeAndReset()
+
+
+def test_config4_sample_batch_matches_per_mesh():
+    spec = g.config4(n_meshes=48, seed=4)
+    dev = [MeshInfo.from_host(h) for h in spec.meshes]
+    try:
+        with Batch(dev, spec.levels, LAK) as b:
+            b.run(use_graph=True)
+            inf = b.info()
+            assert inf.nmeshes == 48 and inf.max_levels == max(spec.levels)
+            tot = 0
+            for i, (h, lv) in enumerate(zip(spec.meshes, spec.levels)):
+                ids, verts, _, _ = fetch(b.result(i))
+                single, _ = subdivide_host(h, lv, LAK)
+                np.testing.assert_array_equal(ids, single.ids)
+                np.testing.assert_array_equal(verts, single.verts)
+                tot += single.nfaces
+            assert inf.out_faces == tot
+    finally:
+        for d in dev:
+            d.freeAndReset()
+
+
+def test_batch_reports_the_invalid_mesh():
+    hosts = [g.cube(), g.tetrahedron(), g.torus(6, 5)]
+    bad = g.HostMesh(hosts[2].type, hosts[2].ptr, hosts[2].ids.copy(), hosts[2].vals, hosts[2].verts)
+    bad.ids[7] = 10_000
+    hosts[2] = bad
+    dev = [MeshInfo.from_host(h) for h in hosts]
+    try:
+        with pytest.raises(SlakError) as ei:
+            Batch(dev, [1, 1, 1], LAK)
+        assert "mesh 2" in str(ei.value)
+    finally:
+        for d in dev:
+            d.freeAndReset()
