@@ -133,6 +133,137 @@ def cpu_baseline(workload, budget_s):
     }
 
 
+def batched_vs_plain(n_meshes, steps, scheme, seed=4):
+    """Batched uneven workload: n_meshes seeded tori (dims uniform in [4,64]^2,
+    20% of quads triangulated, depth uniform in {1..4}). Plain = one
+    subdivideIteratively call per mesh; batched = one launch per kernel per
+    level over the union, replayed as a hipGraph. Outputs are identical."""
+    from slak_amd import capi, meshgen
+    from slak_amd.batch import Batch
+    from slak_amd.mesh import MeshInfo, _subdivide
+
+    t_gen = time.perf_counter()
+    spec = meshgen.config4(n_meshes=n_meshes, seed=seed)
+    gen_s = time.perf_counter() - t_gen
+    dev = [MeshInfo.from_host(h) for h in spec.meshes]
+    try:
+        # plain: per-mesh calls (one warm-up pass over a slice first)
+        for d, lv in list(zip(dev, spec.levels))[:64]:
+            r = MeshInfo()
+            _subdivide(scheme, d, r, lv)
+            r.freeAndReset()
+        capi.check(capi.lib().slak_device_synchronize())
+        t0 = time.perf_counter()
+        faces = 0
+        for d, lv in zip(dev, spec.levels):
+            r = MeshInfo()
+            st = _subdivide(scheme, d, r, lv)
+            faces += int(st.out_faces)
+            r.freeAndReset()
+        capi.check(capi.lib().slak_device_synchronize())
+        plain_s = time.perf_counter() - t0
+        # batched: create (validation + first-level analysis + allocation), graph replays
+        t0 = time.perf_counter()
+        b = Batch(dev, spec.levels, scheme)
+        create_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        b.run(use_graph=True)  # first run captures the graph
+        first_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.run(use_graph=True)
+        batched_s = (time.perf_counter() - t0) / steps
+        inf = b.info()
+        assert int(inf.out_faces) == faces
+        capi.profile_enable(True)
+        b.run(use_graph=False)
+        prof = capi.profile_report()
+        capi.profile_enable(False)
+        b.close()
+    finally:
+        for d in dev:
+            d.freeAndReset()
+    rl = roofline(prof)
+    return {
+        "workload": f"{n_meshes} seeded tori, dims U[4,64]^2, 20% triangulated, depth U{{1..4}}",
+        "out_faces": faces,
+        "plain_per_mesh_s": round(plain_s, 4),
+        "batched_graph_s": round(batched_s, 5),
+        "batched_create_s": round(create_s, 4),
+        "batched_first_run_s": round(first_s, 4),
+        "speedup_vs_plain": round(plain_s / batched_s, 2),
+        "speedup_vs_plain_incl_create": round(plain_s / (create_s + first_s), 2),
+        "faces_per_s_batched": round(faces / batched_s, 1),
+        "faces_per_s_plain": round(faces / plain_s, 1),
+        "roofline": rl,
+        "generation_s": round(gen_s, 2),
+    }
+
+
+def serve_poisson(n_requests, qps_fraction, scheme, seed=5):
+    """Served stream: Poisson arrivals of requests (torus, faces log-uniform in
+    [100, 1e6], 10% triangles, level uniform in {1..3}); latency from submit
+    until the refined mesh is resident. QPS is stated as a fraction of the
+    saturation rate measured with every request submitted at once."""
+    import numpy as np
+
+    from slak_amd import meshgen
+    from slak_amd.mesh import MeshInfo
+    from slak_amd.serve import Server
+
+    reqs = [meshgen.request_mesh(seed, i) for i in range(n_requests)]
+    dev = [MeshInfo.from_host(m) for m, _ in reqs]
+    try:
+        def run(qps):
+            rng = np.random.default_rng(seed)
+            gaps = rng.exponential(1.0 / qps, size=n_requests) if qps else np.zeros(n_requests)
+            lat, faces = [], 0
+            with Server(scheme) as srv:
+                t0 = time.perf_counter()
+                ids = []
+                t_next = t0
+                for i, (d, (m, lv)) in enumerate(zip(dev, reqs)):
+                    t_next += gaps[i]
+                    now = time.perf_counter()
+                    if t_next > now:
+                        time.sleep(t_next - now)
+                    ids.append(srv.submit(d, lv))
+                for rid in ids:
+                    r, st = srv.wait(rid)
+                    lat.append(st.latency_us / 1e3)
+                    faces += int(st.out_faces)
+                    r.freeAndReset()
+                wall = time.perf_counter() - t0
+                sst = srv.stats()
+            lat = np.array(lat)
+            return {
+                "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "mean_ms": round(float(lat.mean()), 3),
+                "faces_per_s": round(faces / wall, 1),
+                "wall_s": round(wall, 3),
+                "steps": int(sst.steps),
+                "max_batch": int(sst.max_batch),
+                "out_faces": faces,
+            }
+
+        sat = run(0.0)  # everything at once: saturation throughput
+        sat_qps = n_requests / sat["wall_s"]
+        qps = qps_fraction * sat_qps
+        loaded = run(qps)
+    finally:
+        for d in dev:
+            d.freeAndReset()
+    return {
+        "workload": f"{n_requests} Poisson requests, faces log-uniform [100, 1e6], 10% triangles, level U{{1..3}}",
+        "qps": round(qps, 2),
+        "qps_fraction_of_saturation": qps_fraction,
+        "saturation_qps": round(sat_qps, 2),
+        **loaded,
+        "saturation": sat,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +273,11 @@ def main():
     ap.add_argument("--scheme", default="lak", choices=["lak", "slak"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--batch-meshes", type=int, default=4096)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-qps-fraction", type=float, default=0.5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (meshes are independent; "
@@ -187,6 +323,10 @@ def main():
         "host_syncs_per_step": int(st.host_syncs),
         "peak_device_bytes": int(st.peak_bytes),
     }
+    if not args.no_batch:
+        line["batched"] = batched_vs_plain(args.batch_meshes, max(3, args.steps // 4), scheme)
+    if not args.no_serve:
+        line["serve"] = serve_poisson(args.serve_requests, args.serve_qps_fraction, scheme)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_budget)
     print(json.dumps(line), flush=True)

@@ -153,6 +153,61 @@ int slak_batch_result(slak_batch* batch, int32_t index, slak_mesh* rmesh);
 int slak_batch_info_get(slak_batch* batch, slak_batch_info* info);
 int slak_batch_destroy(slak_batch* batch);
 
+/* ---- serving (no counterpart in the reference) ----
+ * A request loop with continuous batching: submitted requests are admitted
+ * (FIFO) while the sum of their projected device footprints stays within
+ * memory_budget_bytes and at most max_batch are in flight; every step
+ * advances each admitted request by one level, all of them in the same
+ * batched kernels, so requests that arrive while others are on the GPU join
+ * at the next step. Intermediate levels stay resident on the device. The
+ * submitted mesh must stay valid until the request completes. Results are
+ * identical to slak_subdivide of the same mesh. */
+typedef struct slak_server slak_server;
+
+typedef struct slak_server_config {
+  int32_t scheme;
+  int32_t device;
+  int32_t max_batch;              /* <= 0: 1024 */
+  int32_t reserved;
+  uint64_t memory_budget_bytes;   /* 0: 64 GiB */
+} slak_server_config;
+
+typedef struct slak_request_stats {
+  int32_t status;
+  int32_t levels;
+  int32_t steps;       /* batched steps the request took part in */
+  int32_t reserved;
+  double submit_us;    /* times relative to the server start */
+  double admit_us;
+  double done_us;      /* result resident on the device */
+  double latency_us;   /* done - submit */
+  uint64_t out_faces;
+} slak_request_stats;
+
+typedef struct slak_server_stats {
+  uint64_t steps;
+  uint64_t completed;
+  uint64_t pending;
+  uint64_t active;
+  uint64_t inflight_bytes;
+  uint64_t faces_done;
+  double busy_us;      /* host time inside batched steps */
+  int32_t max_batch;   /* largest batch seen */
+  int32_t reserved;
+} slak_server_stats;
+
+int slak_server_start(const slak_server_config* cfg, slak_server** server);
+/* Returns immediately with a request id; SLAK_ERR_REJECTED when the request
+ * alone exceeds the memory budget or the server is stopping. */
+int slak_server_submit(slak_server* server, const slak_mesh* mesh, int32_t target_level, uint64_t* request_id);
+/* Waits (timeout_ms < 0: forever) for a request; on success *result receives
+ * its refined mesh, owned by the caller (slak_free), in the slak_subdivide form. */
+int slak_server_wait(slak_server* server, uint64_t request_id, int32_t timeout_ms, slak_mesh* result,
+                     slak_request_stats* stats);
+int slak_server_get_stats(slak_server* server, slak_server_stats* stats);
+/* Finishes every submitted request, then stops; uncollected results are freed. */
+int slak_server_stop(slak_server* server);
+
 #ifdef __cplusplus
 }
 #endif

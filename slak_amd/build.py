@@ -17,7 +17,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libslak.so")
 OBJDIR = os.path.join(ROOT, "_build", "slak")
 
-SOURCES = ["slak_kernels.hip", "slak_runtime.hip", "slak_plan.hip", "slak_batch.hip"]
+SOURCES = ["slak_kernels.hip", "slak_runtime.hip", "slak_plan.hip", "slak_batch.hip", "slak_server.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [

@@ -60,6 +60,44 @@ class SlakBatchInfo(ctypes.Structure):
     ]
 
 
+class SlakServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("scheme", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+        ("max_batch", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("memory_budget_bytes", ctypes.c_uint64),
+    ]
+
+
+class SlakRequestStats(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("levels", ctypes.c_int32),
+        ("steps", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("submit_us", ctypes.c_double),
+        ("admit_us", ctypes.c_double),
+        ("done_us", ctypes.c_double),
+        ("latency_us", ctypes.c_double),
+        ("out_faces", ctypes.c_uint64),
+    ]
+
+
+class SlakServerStats(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("pending", ctypes.c_uint64),
+        ("active", ctypes.c_uint64),
+        ("inflight_bytes", ctypes.c_uint64),
+        ("faces_done", ctypes.c_uint64),
+        ("busy_us", ctypes.c_double),
+        ("max_batch", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 class SlakError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"slak error {code}: {msg}")
@@ -92,6 +130,13 @@ EXPORTS = [
     ("slak_batch_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(SlakMesh)]),
     ("slak_batch_info_get", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SlakBatchInfo)]),
     ("slak_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("slak_server_start", ctypes.c_int, [ctypes.POINTER(SlakServerConfig), ctypes.POINTER(ctypes.c_void_p)]),
+    ("slak_server_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SlakMesh), ctypes.c_int32,
+                                          ctypes.POINTER(ctypes.c_uint64)]),
+    ("slak_server_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(SlakMesh),
+                                        ctypes.POINTER(SlakRequestStats)]),
+    ("slak_server_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SlakServerStats)]),
+    ("slak_server_stop", ctypes.c_int, [ctypes.c_void_p]),
 ]
 
 _lib: Optional[ctypes.CDLL] = None

@@ -262,14 +262,23 @@ void Plan::prepare() {
     for (int k = 0; k < nseg; ++k) jobs_[L1.jobs[k]].bad_flags = mf[k];
     throw Error(SLAK_ERR_INVALID_MESH, describe_failure(fl));
   };
-  fail_if(readback(nullptr, 0, nullptr));
+  bool all_trusted = true, all_known = true;
+  for (int k = 0; k < nseg; ++k) {
+    all_trusted &= jobs_[L1.jobs[k]].trusted;
+    all_known &= jobs_[L1.jobs[k]].known_ne >= 0;
+  }
+  if (!all_trusted) fail_if(readback(nullptr, 0, nullptr));
 
   // 2. first-level analysis: M^T, owned-edge counts, per-mesh edge totals
   record_analysis(L1, true);
   d_bounds_ = arena_.get_n<int32_t>(nseg + 1);
   launch_seg_bounds(L1.d_vbase, nseg, L1.eptr, d_bounds_, st_);
-  std::vector<int64_t> bounds(nseg + 1);
-  fail_if(readback(d_bounds_, nseg + 1, bounds.data()));
+  std::vector<int64_t> bounds(nseg + 1, 0);
+  if (!all_known) {
+    fail_if(readback(d_bounds_, nseg + 1, bounds.data()));
+  } else {
+    for (int k = 0; k < nseg; ++k) bounds[k + 1] = bounds[k] + jobs_[L1.jobs[k]].known_ne;
+  }
 
   // 3. closed-form sizes of every later level
   int Lmax = 0;
@@ -299,7 +308,7 @@ void Plan::prepare() {
   }
 
   // 4. level unions, destinations and outputs
-  const bool lak_form = scheme_ == kLAK;
+  const bool lak_scheme = scheme_ == kLAK;
   auto out_alloc = [&](size_t bytes) -> void* {
     if (mode_ == OutputMode::kSeparate) return pool().alloc(bytes);
     return nullptr;  // carved from pooled blocks below
@@ -317,7 +326,7 @@ void Plan::prepare() {
   if (mode_ == OutputMode::kPooled) {
     pool_v = arena_.get_n<float4>(tot_v);
     pool_i = arena_.get_n<int32_t>(tot_i);
-    if (lak_form) {
+    if (lak_scheme) {
       pool_p = arena_.get_n<int32_t>(tot_p);
       pool_a = arena_.get_n<int32_t>(tot_i);
     }
@@ -326,6 +335,7 @@ void Plan::prepare() {
   for (auto& J : jobs_) {
     if (J.levels <= 0) continue;
     int64_t nvo = J.nv[J.levels], nio = 4 * J.nnz[J.levels - 1], npo = J.nf[J.levels] + 1;
+    const bool lak_form = lak_scheme && J.lak_form;
     if (mode_ == OutputMode::kSeparate) {
       J.out_verts = static_cast<float4*>(out_alloc(nvo * sizeof(float4)));
       J.out_ids = static_cast<int32_t*>(out_alloc(nio * sizeof(int32_t)));

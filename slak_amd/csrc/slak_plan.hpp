@@ -21,6 +21,11 @@ struct MeshJob {
   slak_mesh in{};   // caller's device mesh (never modified)
   int target = 0;   // requested level
   int levels = 0;   // levels to refine (a non-quad input gets at least one)
+  // A mesh this library refined itself: valid by construction (no validation
+  // round trip) and with its edge count known in closed form (known_ne >= 0).
+  bool trusted = false;
+  int64_t known_ne = -1;
+  bool lak_form = true;  // LAK scheme: also write ptr/vals of the final level
   // closed-form sizes of the input of level l (l = 0..levels) and its edges
   std::vector<int64_t> nv, nf, nnz, ne;
   // final outputs
