@@ -289,6 +289,9 @@ def main():
     desc, levels = WORKLOADS[args.workload]
     mesh = make_mesh(args.workload)
     scheme = capi.SLAK_SCHEME_LAK if args.scheme == "lak" else capi.SLAK_SCHEME_SLAK
+    import ctypes
+    copy_gbps = ctypes.c_double()
+    capi.check(capi.lib().slak_copy_probe(1 << 30, 5, ctypes.byref(copy_gbps)))
     secs, faces, st, prof = time_gpu(mesh, levels, scheme, args.steps, args.warmup)
     other = capi.SLAK_SCHEME_SLAK if scheme == capi.SLAK_SCHEME_LAK else capi.SLAK_SCHEME_LAK
     secs_o, _, _, _ = time_gpu(mesh, levels, other, max(3, args.steps // 2), 1)
@@ -318,6 +321,7 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": roofline(prof),
+        "copy_kernel_gbps": round(copy_gbps.value, 1),
         "other_scheme_faces_per_s": round(faces * max(3, args.steps // 2) / secs_o, 1),
         "stages": prof["stages"],
         "host_syncs_per_step": int(st.host_syncs),

@@ -102,6 +102,10 @@ int slak_pool_stats(uint64_t* in_use, uint64_t* cached, uint64_t* peak);
  * bytes are the stage's algorithmic (compulsory) bytes summed over launches. */
 int slak_profile_enable(int32_t on);
 int slak_profile_report(char* buf, uint64_t len);
+/* HBM reference: `iters` float4 stream copies of `bytes` (k_copy_probe);
+ * *gbps = (read + write bytes) / time. Also the calibration kernel for the
+ * FETCH_SIZE / WRITE_SIZE counters. */
+int slak_copy_probe(uint64_t bytes, int32_t iters, double* gbps);
 
 /* ---- sparse operators of the LAK method (LibInclude/math/bhblas.h:30-51) ---- */
 

@@ -122,6 +122,7 @@ EXPORTS = [
      + [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), ctypes.c_void_p]),
     ("slak_profile_enable", ctypes.c_int, [ctypes.c_int32]),
     ("slak_profile_report", ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64]),
+    ("slak_copy_probe", ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
     ("slak_scan_exclusive", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
     ("slak_batch_create", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(SlakMesh),
                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_void_p),

@@ -39,13 +39,17 @@ void launch_fill_i32(int32_t* p, int64_t n, int32_t value, hipStream_t st);
 void launch_gather(const SegTable& seg, int32_t nf_total, bool use_vals, int32_t* ids_pos, int32_t* fptr,
                    int32_t* cface, int* flags, hipStream_t st);
 void launch_gather_verts(const SegTable& seg, int32_t nv_total, float4* P, hipStream_t st);
+constexpr int kRowHistBins = 17;  // row lengths 0..15, 16 = longer
+void launch_row_hist(const int32_t* tptr, int32_t nv, const int32_t* fptr, int32_t nf, int32_t face_order,
+                     int32_t* hist, hipStream_t st);
+void launch_copy_probe(const float4* a, float4* b, int64_t n, hipStream_t st);
 void launch_seg_bounds(const int64_t* vbase, int32_t n, const int32_t* eptr, int32_t* out, hipStream_t st);
 void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int32_t* fptr, const float4* P, int32_t nf,
                   const SegTable& seg, int32_t* cnt, hipStream_t st);
 void launch_transpose(bool quad, const int32_t* ids, const int32_t* fptr, int32_t nf, const int32_t* tptr,
                       int32_t* cnt, int32_t* tids, hipStream_t st);
-void launch_symbolic(bool quad, int scheme, const LevelView& L, hipStream_t st);
-void launch_numeric(bool quad, int scheme, const LevelView& L, hipStream_t st);
+void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
+void launch_numeric(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
 void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st);
 void launch_topology(bool quad, const int32_t* ids, const int32_t* fptr, const int32_t* eout, int32_t nf,
                      const SegTable& seg, hipStream_t st);

@@ -17,6 +17,8 @@
 // so every output is bitwise deterministic.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "slak_device.hpp"
 #include "slak_internal.hpp"
 
@@ -39,6 +41,16 @@ __global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t value) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) p[i] = value;
+}
+
+// ---------------------------------------------------------------------------
+// HBM reference: a float4 stream copy (bandwidth probe and counter calibration)
+
+__global__ void __launch_bounds__(kBlock) k_copy_probe(const float4* __restrict__ a, float4* __restrict__ b,
+                                                        int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) b[i] = a[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -111,6 +123,31 @@ __global__ void k_gather_verts(SegTable seg, int32_t nv_total, float4* __restric
   const int k = seg_of(seg.vbase, seg.n, v);
   const SegDesc* S = seg.d + k;
   P[v] = S->src_verts[v - S->vb];
+}
+
+// Row-length histograms of a level: M^T rows (valence, bins 0..15, 16 = more)
+// and face orders (same bins), block-local in LDS then one atomic per bin.
+constexpr int kHistBins = 17;
+
+__global__ void __launch_bounds__(kBlock) k_row_hist(const int32_t* __restrict__ tptr, int32_t nv,
+                                                      const int32_t* __restrict__ fptr, int32_t nf, int32_t face_order,
+                                                      int32_t* __restrict__ hist) {
+  __shared__ int32_t h[2 * kHistBins];
+  for (int i = threadIdx.x; i < 2 * kHistBins; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = i; v < nv; v += stride) {
+    int n = tptr[v + 1] - tptr[v];
+    atomicAdd(&h[n < kHistBins - 1 ? n : kHistBins - 1], 1);
+  }
+  for (int64_t f = i; f < nf; f += stride) {
+    int n = fptr ? fptr[f + 1] - fptr[f] : face_order;
+    atomicAdd(&h[kHistBins + (n < kHistBins - 1 ? n : kHistBins - 1)], 1);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < 2 * kHistBins; b += blockDim.x)
+    if (h[b]) atomicAdd(&hist[b], h[b]);
 }
 
 // eptr at every segment's first vertex: out[k] = eptr[vbase[k]], k = 0..n
@@ -193,7 +230,7 @@ __device__ void sort_row_in_place(int32_t* a, int n) {
   }
 }
 
-template <bool QUAD, int SCHEME>
+template <bool QUAD, int SCHEME, int CAP>
 __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
   int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= L.nv) return;
@@ -201,8 +238,8 @@ __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
   int n = L.tptr[v + 1] - s;
   int32_t count = 0;
   int bad = 0;
-  if (n <= kRowCap) {
-    RegRow<kRowCap, QUAD> r;
+  if (n <= CAP) {
+    RegRow<CAP, QUAD> r;
     r.load(L, s, n);
     row_symbolic<SCHEME>(r, v, count, bad);
   } else {
@@ -217,14 +254,14 @@ __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
   }
 }
 
-template <bool QUAD, int SCHEME>
+template <bool QUAD, int SCHEME, int CAP>
 __global__ void __launch_bounds__(kBlock) k_numeric(LevelView L) {
   int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= L.nv) return;
   int32_t s = L.tptr[v];
   int n = L.tptr[v + 1] - s;
-  if (n <= kRowCap) {
-    RegRow<kRowCap, QUAD> r;
+  if (n <= CAP) {
+    RegRow<CAP, QUAD> r;
     r.load(L, s, n);
     row_numeric<SCHEME>(r, L, v);
   } else {
@@ -414,6 +451,19 @@ void launch_gather_verts(const SegTable& seg, int32_t nv_total, float4* P, hipSt
   k_gather_verts<<<grid_for(nv_total), kBlock, 0, st>>>(seg, nv_total, P);
 }
 
+void launch_row_hist(const int32_t* tptr, int32_t nv, const int32_t* fptr, int32_t nf, int32_t face_order,
+                     int32_t* hist, hipStream_t st) {
+  int64_t g = (std::max<int64_t>(nv, nf) + kBlock - 1) / kBlock;
+  if (g > 1024) g = 1024;
+  if (g < 1) g = 1;
+  k_fill_i32<<<1, 64, 0, st>>>(hist, 2 * kHistBins, 0);
+  k_row_hist<<<(unsigned)g, kBlock, 0, st>>>(tptr, nv, fptr, nf, face_order, hist);
+}
+
+void launch_copy_probe(const float4* a, float4* b, int64_t n, hipStream_t st) {
+  k_copy_probe<<<8192, kBlock, 0, st>>>(a, b, n);
+}
+
 void launch_seg_bounds(const int64_t* vbase, int32_t n, const int32_t* eptr, int32_t* out, hipStream_t st) {
   k_seg_bounds<<<grid_for(n + 1), kBlock, 0, st>>>(vbase, n, eptr, out);
 }
@@ -444,28 +494,39 @@ void launch_transpose(bool quad, const int32_t* ids, const int32_t* fptr, int32_
     k_transpose<false><<<grid_for(nf), kBlock, 0, st>>>(ids, fptr, nf, tptr, cnt, tids);
 }
 
-void launch_symbolic(bool quad, int scheme, const LevelView& L, hipStream_t st) {
+// cap: M^T row length held in registers (4 or 8); longer rows take the
+// in-memory path. Chosen per level from the row-length histogram.
+#define SLAK_ROW_DISPATCH(KERNEL, quad, scheme, cap, g, L, st)                       \
+  do {                                                                               \
+    if (quad) {                                                                      \
+      if (scheme == kLAK) {                                                          \
+        if (cap <= 4) KERNEL<true, kLAK, 4><<<g, kBlock, 0, st>>>(L);                \
+        else KERNEL<true, kLAK, 8><<<g, kBlock, 0, st>>>(L);                         \
+      } else {                                                                       \
+        if (cap <= 4) KERNEL<true, kSLAK, 4><<<g, kBlock, 0, st>>>(L);               \
+        else KERNEL<true, kSLAK, 8><<<g, kBlock, 0, st>>>(L);                        \
+      }                                                                              \
+    } else {                                                                         \
+      if (scheme == kLAK) {                                                          \
+        if (cap <= 4) KERNEL<false, kLAK, 4><<<g, kBlock, 0, st>>>(L);               \
+        else KERNEL<false, kLAK, 8><<<g, kBlock, 0, st>>>(L);                        \
+      } else {                                                                       \
+        if (cap <= 4) KERNEL<false, kSLAK, 4><<<g, kBlock, 0, st>>>(L);              \
+        else KERNEL<false, kSLAK, 8><<<g, kBlock, 0, st>>>(L);                       \
+      }                                                                              \
+    }                                                                                \
+  } while (0)
+
+void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st) {
   if (L.nv <= 0) return;
   unsigned g = grid_for(L.nv);
-  if (quad) {
-    if (scheme == kLAK) k_symbolic<true, kLAK><<<g, kBlock, 0, st>>>(L);
-    else k_symbolic<true, kSLAK><<<g, kBlock, 0, st>>>(L);
-  } else {
-    if (scheme == kLAK) k_symbolic<false, kLAK><<<g, kBlock, 0, st>>>(L);
-    else k_symbolic<false, kSLAK><<<g, kBlock, 0, st>>>(L);
-  }
+  SLAK_ROW_DISPATCH(k_symbolic, quad, scheme, cap, g, L, st);
 }
 
-void launch_numeric(bool quad, int scheme, const LevelView& L, hipStream_t st) {
+void launch_numeric(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st) {
   if (L.nv <= 0) return;
   unsigned g = grid_for(L.nv);
-  if (quad) {
-    if (scheme == kLAK) k_numeric<true, kLAK><<<g, kBlock, 0, st>>>(L);
-    else k_numeric<true, kSLAK><<<g, kBlock, 0, st>>>(L);
-  } else {
-    if (scheme == kLAK) k_numeric<false, kLAK><<<g, kBlock, 0, st>>>(L);
-    else k_numeric<false, kSLAK><<<g, kBlock, 0, st>>>(L);
-  }
+  SLAK_ROW_DISPATCH(k_numeric, quad, scheme, cap, g, L, st);
 }
 
 void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st) {

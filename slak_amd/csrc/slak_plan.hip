@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "slak_plan.hpp"
@@ -49,6 +50,26 @@ void check_i32(int64_t x, const std::string& what) {
 }
 
 }  // namespace
+
+// Register capacity of the SpGEMM row kernels for a level with the given
+// row-length statistics. Rows above the capacity take the in-memory path
+// (correct for any length, slower); capacity 4 keeps the row state in half
+// the registers, which doubles occupancy. Pick 4 when every row fits or the
+// long rows are rare enough that the waves they stall cost less than the
+// occupancy gained; the threshold is a tuned constant (SLAK_CAP_THRESHOLD
+// overrides it, SLAK_ROW_CAP forces a capacity).
+int autotune_cap(bool quad, int64_t nv, int64_t long_rows, int max_row) {
+  const char* ef = std::getenv("SLAK_ROW_CAP");
+  const char* et = std::getenv("SLAK_CAP_THRESHOLD");
+  const int forced = ef ? std::atoi(ef) : 0;
+  const double thresh = et ? std::atof(et) : 0.02;
+  if (forced == 4 || forced == 8) return forced;
+  if (max_row <= 4) return 4;
+  if (nv <= 0) return 8;
+  double frac = (double)long_rows / (double)nv;
+  (void)quad;
+  return frac < thresh ? 4 : 8;
+}
 
 Plan::Plan(int scheme, std::vector<MeshJob> jobs, OutputMode mode, hipStream_t st)
     : scheme_(scheme), jobs_(std::move(jobs)), mode_(mode), st_(st), arena_(pool()) {}
@@ -205,7 +226,7 @@ void Plan::record_analysis(LevelPlan& L, bool count) {
   V.seg = L.table();
   {
     ProfScope ps(st_, L.quad ? "spgemm_symbolic_quad" : "spgemm_symbolic_poly", 8.0 * L.nnz + 8.0 * L.nv + poly_b);
-    launch_symbolic(L.quad, scheme_, V, st_);
+    launch_symbolic(L.quad, scheme_, L.cap, V, st_);
   }
   {
     ProfScope ps(st_, "scan", 8.0 * L.nv + 4);
@@ -271,13 +292,33 @@ void Plan::prepare() {
 
   // 2. first-level analysis: M^T, owned-edge counts, per-mesh edge totals
   record_analysis(L1, true);
-  d_bounds_ = arena_.get_n<int32_t>(nseg + 1);
+  d_bounds_ = arena_.get_n<int32_t>(nseg + 1 + 2 * kRowHistBins);
   launch_seg_bounds(L1.d_vbase, nseg, L1.eptr, d_bounds_, st_);
-  std::vector<int64_t> bounds(nseg + 1, 0);
+  launch_row_hist(L1.tptr, (int32_t)L1.nv, L1.fptr, (int32_t)L1.nf, L1.quad ? 4 : 0, d_bounds_ + nseg + 1, st_);
+  std::vector<int64_t> bounds(nseg + 1 + 2 * kRowHistBins, 0);
+  bool have_hist = false;
   if (!all_known) {
-    fail_if(readback(d_bounds_, nseg + 1, bounds.data()));
+    fail_if(readback(d_bounds_, nseg + 1 + 2 * kRowHistBins, bounds.data()));
+    have_hist = true;
   } else {
     for (int k = 0; k < nseg; ++k) bounds[k + 1] = bounds[k] + jobs_[L1.jobs[k]].known_ne;
+  }
+  // row-length statistics of every level, predicted from the first level:
+  // original vertices keep their valence, face points take the face order,
+  // edge points have 4 (2 on the boundary), later face points 4.
+  int64_t vlong = 0, flong = 0;
+  int vmax = 0, fmax = 0;
+  if (have_hist) {
+    const int64_t* hv = bounds.data() + nseg + 1;
+    const int64_t* hf = hv + kRowHistBins;
+    for (int b = 0; b < kRowHistBins; ++b) {
+      if (b > 4) {
+        vlong += hv[b];
+        flong += hf[b];
+      }
+      if (hv[b]) vmax = b;
+      if (hf[b]) fmax = b;
+    }
   }
 
   // 3. closed-form sizes of every later level
@@ -456,6 +497,16 @@ void Plan::prepare() {
       L.eout = eout;
     }
   }
+  for (int l = 1; l <= Lmax; ++l) {
+    LevelPlan& L = levels_[l - 1];
+    if (!have_hist) {
+      L.cap = 8;
+      continue;
+    }
+    L.long_rows = l == 1 ? vlong : vlong + flong;
+    L.max_row = l == 1 ? vmax : std::max(std::max(vmax, fmax), 4);
+    L.cap = autotune_cap(L.quad, L.nv, L.long_rows, L.max_row);
+  }
   for (auto& L : levels_) upload_tables(L);
   prepared_ = true;
 }
@@ -505,7 +556,7 @@ void Plan::record_level(int l) {
     ProfScope ps(st_, L.quad ? "spgemm_numeric_quad" : "spgemm_numeric_poly",
                  8.0 * L.nv + 8.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 16.0 * L.nv + 16.0 * ne_tot + 4.0 * L.nnz +
                      4.0 * L.nv + poly_b);
-    launch_numeric(L.quad, scheme_, V, st_);
+    launch_numeric(L.quad, scheme_, L.cap, V, st_);
   }
   {
     bool lak_final = false;

@@ -43,6 +43,9 @@ struct LevelPlan {
   const int32_t* fptr = nullptr;
   const int32_t* cface = nullptr;
   const float4* P = nullptr;
+  int cap = 8;              // M^T row length held in registers (autotuned per level)
+  int64_t long_rows = -1;   // rows longer than 4 (predicted from the first level's histogram)
+  int max_row = -1;
   // scratch (the first level owns its own; later levels share the largest)
   int32_t *cnt = nullptr, *tptr = nullptr, *tids = nullptr, *ecnt = nullptr, *eptr = nullptr, *eout = nullptr;
   std::vector<int> jobs;  // segment -> job

@@ -432,6 +432,30 @@ int slak_profile_report(char* buf, uint64_t len) {
   });
 }
 
+int slak_copy_probe(uint64_t bytes, int32_t iters, double* gbps) {
+  return guarded([&] {
+    if (!gbps || iters <= 0 || bytes < 16) throw Error(SLAK_ERR_INVALID_ARGUMENT, "bad argument");
+    slak::Arena arena(slak::pool());
+    const int64_t n = (int64_t)(bytes / 16);
+    float4* a = arena.get_n<float4>(n);
+    float4* b = arena.get_n<float4>(n);
+    SLAK_HIP_CHECK(hipMemsetAsync(a, 0, n * 16, nullptr));
+    slak::launch_copy_probe(a, b, n, nullptr);  // warm-up
+    hipEvent_t e0, e1;
+    SLAK_HIP_CHECK(hipEventCreate(&e0));
+    SLAK_HIP_CHECK(hipEventCreate(&e1));
+    SLAK_HIP_CHECK(hipEventRecord(e0, nullptr));
+    for (int i = 0; i < iters; ++i) slak::launch_copy_probe(a, b, n, nullptr);
+    SLAK_HIP_CHECK(hipEventRecord(e1, nullptr));
+    SLAK_HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    SLAK_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbps = 2.0 * 16.0 * (double)n * iters / (ms * 1e-3) / 1e9;
+  });
+}
+
 int slak_scan_exclusive(const int32_t* in, int32_t* out, uint64_t n, void* stream) {
   return guarded([&] {
     if ((!in && n) || !out) throw Error(SLAK_ERR_INVALID_ARGUMENT, "null pointer");

@@ -249,3 +249,16 @@ def test_invalid_mesh_rejected(kind):
     # the library stays usable after a rejected call
     ok, _ = subdivide_host(g.cube(), 1, LAK)
     assert ok.nfaces == 24
+
+
+@pytest.mark.parametrize("cap", ["4", "8"])
+def test_row_capacity_variants_bitwise_equal(monkeypatch, cap):
+    """Both register capacities (rows above it take the in-memory path) give
+    the same bits; cfg2 has valence-5..8 rows that exercise both paths."""
+    m = g.torus_mixed(64, 32, 0.25, pentagons=True, seed=12)
+    monkeypatch.delenv("SLAK_ROW_CAP", raising=False)
+    ref, _ = subdivide_host(m, 3, LAK)
+    monkeypatch.setenv("SLAK_ROW_CAP", cap)
+    got, _ = subdivide_host(m, 3, LAK)
+    np.testing.assert_array_equal(got.ids, ref.ids)
+    np.testing.assert_array_equal(got.verts, ref.verts)
