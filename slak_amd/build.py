@@ -27,6 +27,9 @@ FLAGS = [
     "-fPIC",
     "-Wall",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no FMA contraction: every code path rounds the same way, so results are
+    # bitwise identical across kernel variants, batches and single calls
+    "-ffp-contract=off",
     "-I", os.path.join(ROOT, "include"),
     "-I", CSRC,
 ]

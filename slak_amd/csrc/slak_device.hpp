@@ -45,6 +45,9 @@ struct SegDesc {
   int32_t* dst_ptr;        // LAK form (final level) or null
   int32_t* dst_vals;       // LAK form (final level) or null
   int64_t id_offset;       // added to local refined ids
+  // next level's union bases of this mesh (-1: the mesh finishes here); the
+  // level writes the refined mesh's M^T rows there directly
+  int64_t next_vb, next_cb;
   // first-level sources (the caller's mesh)
   const int32_t* src_ptr;  // null: homogeneous with src_stride
   const int32_t* src_ids;
@@ -93,6 +96,11 @@ struct LevelView {
   int32_t* E_vals;  // optional LAK edge matrix values (+-(id+1))
   int* flags;
   SegTable seg;
+  // refined incidence for the next level (null when no mesh continues)
+  int32_t* ntptr;
+  int32_t* ntids;
+  int32_t* bcnt;         // owned boundary edges per vertex (symbolic), or null
+  const int32_t* bptr;   // its exclusive scan (numeric), or null: no boundary
   // mapped SpGEMM operator outputs (F, D rows share M^T's row offsets)
   int32_t* F_ids;
   int32_t* F_vals;
@@ -314,9 +322,9 @@ struct MemRow {
 //   SLAK owns its flagged out-half-edges v->w: twin present and v < w, or no
 //        twin (CatmullClarkKernels.cuh:147-236).
 template <int SCHEME, class Row>
-__device__ __forceinline__ void row_symbolic(const Row& r, int32_t v, int32_t& count, int& bad) {
+__device__ __forceinline__ void row_symbolic(const Row& r, int32_t v, int32_t& count, int32_t& bcount, int& bad) {
   constexpr int C = Row::kCap;
-  int c = 0;
+  int c = 0, b = 0;
   rloop<C>(r.n, [&](int i) __attribute__((always_inline)) {
     int32_t wi = r.W(i), ui = r.U(i);
     if (wi == v || ui == v) bad |= kFlagDegenerate;
@@ -328,19 +336,27 @@ __device__ __forceinline__ void row_symbolic(const Row& r, int32_t v, int32_t& c
       if (r.U(j) == wi && r.F(j) == r.F(i)) bad |= kFlagDegenerate;
     });
     if (SCHEME == kLAK) {
-      if (wi < v) ++c;
-      if (ui < v && r.IM(i) < 0) ++c;
+      if (wi < v) {
+        ++c;
+        b += r.JM(i) < 0;
+      }
+      if (ui < v && r.IM(i) < 0) {
+        ++c;
+        ++b;
+      }
     } else {
       int jm = r.JM(i);
       if (jm < 0 || wi > v) ++c;
+      b += jm < 0;
     }
   });
   count = c;
+  bcount = b;
 }
 
 // Numeric pass: edge ids, edge points and the vertex point of row v, plus the
 // edge id of every half-edge of the edges v owns (eout).
-template <int SCHEME, class Row>
+template <int SCHEME, bool QUADT, class Row>
 __device__ __forceinline__ void row_numeric(Row& r, const LevelView& L, int32_t v) {
   constexpr int C = Row::kCap;
   // local copies: lambdas below capture locals only, so the kernel argument
@@ -361,6 +377,63 @@ __device__ __forceinline__ void row_numeric(Row& r, const LevelView& L, int32_t 
   const float4 Pv = P[v];
   const int32_t e0 = L.eptr[v] - L.eptr[vb];  // mesh-local edge base of v
   const int n = r.n;
+  // refined incidence of the next level: local corner c of this level maps to
+  // the refined quads 4c..4c+3, i.e. next-level corners ncb + 4c + j
+  const int64_t ncb = S->next_cb, nvb = S->next_vb, cb = S->cb;
+  const int64_t snnz = S->nnz;
+  int32_t* __restrict__ ntptr = L.ntptr;
+  int32_t* __restrict__ ntids = L.ntids;
+  const int32_t* __restrict__ cface = L.cface;
+  const int32_t* __restrict__ fptr = L.fptr;
+  const int32_t bb = L.bptr ? L.bptr[v] - L.bptr[vb] : 0;  // boundary edges owned by earlier vertices
+  auto next_corner = [cface, fptr](int32_t t) __attribute__((always_inline)) {
+    if (QUADT) {
+      return (t & ~3) | ((t + 1) & 3);
+    } else {
+      int32_t f = cface[t];
+      return (t + 1 == fptr[f + 1]) ? fptr[f] : t + 1;
+    }
+  };
+  // row of edge point e: corners (4 ta + 1, 4 next(ta) + 3[, 4 tb + 1, 4 next(tb) + 3]), sorted
+  auto edge_row = [&](int32_t e, int32_t bl, int32_t ta, int32_t tb) __attribute__((always_inline)) {
+    if (ncb < 0 || e >= ne_cap) return;
+    const int64_t row = ncb + 2 * snnz + 4 * (int64_t)e - 2 * (int64_t)bl;
+    ntptr[nvb + snv + snf + e] = (int32_t)row;
+    int32_t a0 = 4 * (int32_t)(ta - cb) + 1, a1 = 4 * (int32_t)(next_corner(ta) - cb) + 3;
+    if (a1 < a0) {
+      int32_t x = a0;
+      a0 = a1;
+      a1 = x;
+    }
+    if (tb < 0) {
+      ntids[row] = (int32_t)(ncb + a0);
+      ntids[row + 1] = (int32_t)(ncb + a1);
+      return;
+    }
+    int32_t b0 = 4 * (int32_t)(tb - cb) + 1, b1 = 4 * (int32_t)(next_corner(tb) - cb) + 3;
+    if (b1 < b0) {
+      int32_t x = b0;
+      b0 = b1;
+      b1 = x;
+    }
+    // merge two sorted pairs
+    int32_t m0 = min(a0, b0), m3 = max(a1, b1);
+    int32_t lo = max(a0, b0), hi = min(a1, b1);
+    int32_t m1 = min(lo, hi), m2 = max(lo, hi);
+    ntids[row] = (int32_t)(ncb + m0);
+    ntids[row + 1] = (int32_t)(ncb + m1);
+    ntids[row + 2] = (int32_t)(ncb + m2);
+    ntids[row + 3] = (int32_t)(ncb + m3);
+  };
+  // LAK: boundary edges among v's owned edges with a smaller neighbour than x
+  auto lak_brank = [&](int32_t x) __attribute__((always_inline)) {
+    int k = 0;
+    rloop<C>(n, [&](int j) __attribute__((always_inline)) {
+      k += (r.W(j) < x) && (r.JM(j) < 0);
+      k += (r.U(j) < x) && (r.IM(j) < 0);
+    });
+    return k;
+  };
   // Edge ids beyond the segment's edge points are clamped (and flagged) so
   // that refined ids stay inside the refined vertex range.
   auto cap = [flags, ne_cap](int32_t e) __attribute__((always_inline)) {
@@ -390,6 +463,7 @@ __device__ __forceinline__ void row_numeric(Row& r, const LevelView& L, int32_t 
           E_ids[e] = x;
           E_vals[e] = jm >= 0 ? -(e + 1) : (e + 1);
         }
+        if (ncb >= 0) edge_row(e, bb + (L.bptr ? lak_brank(x) : 0), r.T(i), jm >= 0 ? r.TPJ(i) : -1);
       }
       int32_t y = r.U(i);
       if (y < v && r.IM(i) < 0) {  // boundary edge seen only as y -> v
@@ -400,16 +474,19 @@ __device__ __forceinline__ void row_numeric(Row& r, const LevelView& L, int32_t 
           E_ids[e] = y;
           E_vals[e] = e + 1;
         }
+        if (ncb >= 0) edge_row(e, bb + (L.bptr ? lak_brank(y) : 0), r.TP(i), -1);
       }
     });
   } else {
-    int kk = 0;
+    int kk = 0, kb = 0;
     rloop<C>(n, [&](int i) __attribute__((always_inline)) {
       int32_t x = r.W(i);
       int jm = r.JM(i);
       if (jm < 0 || x > v) {
         int32_t e = cap(e0 + kk);
         ++kk;
+        if (ncb >= 0) edge_row(e, bb + kb, r.T(i), jm >= 0 ? r.TPJ(i) : -1);
+        kb += jm < 0;
         float4 q;
         if (jm >= 0)  // interior, v < x: f0 = face of v->x, f1 = face of x->v
           q = f4mul(f4add(f4add(f4add(Pv, P[x]), FP(r.F(i))), FP(r.FJ(i))), 0.25f);
@@ -442,6 +519,13 @@ __device__ __forceinline__ void row_numeric(Row& r, const LevelView& L, int32_t 
     });
   }
   dst[v - vb] = q;
+  if (ncb >= 0) {  // row of the vertex itself: corner 0 of each refined quad at v
+    const int64_t row = ncb + (L.tptr[v] - cb);
+    ntptr[nvb + (v - vb)] = (int32_t)row;
+    rloop<C>(n, [&](int i) __attribute__((always_inline)) {
+      ntids[row + i] = (int32_t)(ncb + 4 * (r.T(i) - cb));
+    });
+  }
 }
 
 // Mapped SpGEMM operator for row v (spgemm_mapped + compress): F row =

@@ -35,6 +35,8 @@ struct Error : std::runtime_error {
 // ---------------------------------------------------------------------------
 // launchers (slak_kernels.hip)
 
+// reads SLAK_XCD_REMAP once and sets the device-side option (call before capture)
+void init_device_options();
 void launch_fill_i32(int32_t* p, int64_t n, int32_t value, hipStream_t st);
 void launch_gather(const SegTable& seg, int32_t nf_total, bool use_vals, int32_t* ids_pos, int32_t* fptr,
                    int32_t* cface, int* flags, hipStream_t st);
@@ -52,7 +54,7 @@ void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStre
 void launch_numeric(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
 void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st);
 void launch_topology(bool quad, const int32_t* ids, const int32_t* fptr, const int32_t* eout, int32_t nf,
-                     const SegTable& seg, hipStream_t st);
+                     const SegTable& seg, int32_t* ntptr, int32_t* ntids, hipStream_t st);
 int64_t scan_partials_len(int64_t n);
 void launch_scan(const int32_t* in, int32_t* out, int64_t n, int64_t* partial, int* flags, hipStream_t st);
 

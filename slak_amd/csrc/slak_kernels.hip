@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <mutex>
 
 #include "slak_device.hpp"
 #include "slak_internal.hpp"
@@ -28,6 +30,19 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kRowCap = 8;  // M^T rows up to this length stay in registers
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// With the remap, consecutive tiles (which gather from neighbouring parts of
+// the mesh) land on the same XCD. Selected once per process (SLAK_XCD_REMAP).
+__device__ int g_xcd_remap;
+
+__device__ __forceinline__ int64_t block_index() {
+  const int b = blockIdx.x;
+  if (!g_xcd_remap) return b;
+  const int nb = gridDim.x;
+  const int per = nb >> 3, rem = nb & 7, x = b & 7, i = b >> 3;
+  return (int64_t)((x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + i);
+}
 
 inline unsigned grid_for(int64_t n, int block = kBlock) {
   int64_t g = (n + block - 1) / block;
@@ -59,7 +74,7 @@ __global__ void __launch_bounds__(kBlock) k_copy_probe(const float4* __restrict_
 
 __global__ void k_gather(SegTable seg, int32_t nf_total, int use_vals, int32_t* __restrict__ ids_pos,
                          int32_t* __restrict__ fptr, int32_t* __restrict__ cface, int* __restrict__ flags) {
-  int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (f >= nf_total) return;
   const int k = seg_of(seg.fbase, seg.n, f);
   const SegDesc* S = seg.d + k;
@@ -118,7 +133,7 @@ __global__ void k_gather(SegTable seg, int32_t nf_total, int use_vals, int32_t* 
 }
 
 __global__ void k_gather_verts(SegTable seg, int32_t nv_total, float4* __restrict__ P) {
-  int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t v = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (v >= nv_total) return;
   const int k = seg_of(seg.vbase, seg.n, v);
   const SegDesc* S = seg.d + k;
@@ -164,7 +179,7 @@ __global__ void k_seg_bounds(const int64_t* __restrict__ vbase, int32_t n, const
 template <bool QUAD, bool FP, bool COUNT>
 __global__ void k_faces(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr, const float4* __restrict__ P,
                         int32_t nf, SegTable seg, int32_t* __restrict__ cnt) {
-  int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (f >= nf) return;
   float4* dst = nullptr;
   if (FP) {
@@ -198,7 +213,7 @@ __global__ void k_faces(const int32_t* __restrict__ ids, const int32_t* __restri
 template <bool QUAD>
 __global__ void k_transpose(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr, int32_t nf,
                             const int32_t* __restrict__ tptr, int32_t* __restrict__ cnt, int32_t* __restrict__ tids) {
-  int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (f >= nf) return;
   if (QUAD) {
     int4 c = reinterpret_cast<const int4*>(ids)[f];
@@ -232,22 +247,23 @@ __device__ void sort_row_in_place(int32_t* a, int n) {
 
 template <bool QUAD, int SCHEME, int CAP>
 __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
-  int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t v = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (v >= L.nv) return;
   int32_t s = L.tptr[v];
   int n = L.tptr[v + 1] - s;
-  int32_t count = 0;
+  int32_t count = 0, bcount = 0;
   int bad = 0;
   if (n <= CAP) {
     RegRow<CAP, QUAD> r;
     r.load(L, s, n);
-    row_symbolic<SCHEME>(r, v, count, bad);
+    row_symbolic<SCHEME>(r, v, count, bcount, bad);
   } else {
     sort_row_in_place(L.tids + s, n);
     MemRow<QUAD> r(L, s, n);
-    row_symbolic<SCHEME>(r, v, count, bad);
+    row_symbolic<SCHEME>(r, v, count, bcount, bad);
   }
   L.ecnt[v] = count;
+  if (L.bcnt) L.bcnt[v] = bcount;
   if (bad) {
     atomicOr(L.flags, bad);
     if (L.seg.mflags) atomicOr(L.seg.mflags + seg_of(L.seg.vbase, L.seg.n, v), bad);
@@ -256,23 +272,26 @@ __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
 
 template <bool QUAD, int SCHEME, int CAP>
 __global__ void __launch_bounds__(kBlock) k_numeric(LevelView L) {
-  int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t v = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (v >= L.nv) return;
   int32_t s = L.tptr[v];
   int n = L.tptr[v + 1] - s;
   if (n <= CAP) {
     RegRow<CAP, QUAD> r;
     r.load(L, s, n);
-    row_numeric<SCHEME>(r, L, v);
+    row_numeric<SCHEME, QUAD>(r, L, v);
   } else {
+    // the symbolic pass may have held this row in registers (another
+    // capacity); the in-memory path needs it sorted (idempotent)
+    sort_row_in_place(L.tids + s, n);
     MemRow<QUAD> r(L, s, n);
-    row_numeric<SCHEME>(r, L, v);
+    row_numeric<SCHEME, QUAD>(r, L, v);
   }
 }
 
 template <bool QUAD>
 __global__ void __launch_bounds__(kBlock) k_spgemm_fde(LevelView L) {
-  int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t v = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (v >= L.nv) return;
   int32_t s = L.tptr[v];
   int n = L.tptr[v + 1] - s;
@@ -291,8 +310,8 @@ __global__ void __launch_bounds__(kBlock) k_spgemm_fde(LevelView L) {
 // mesh's local numbering, plus the segment's id offset
 
 __global__ void k_topology_quad(const int32_t* __restrict__ ids, const int32_t* __restrict__ eout, int32_t nf,
-                                SegTable seg) {
-  int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+                                SegTable seg, int32_t* __restrict__ L_ntptr, int32_t* __restrict__ L_ntids) {
+  int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (f >= nf) return;
   const SegDesc* S = seg.d + seg_of(seg.fbase, seg.n, f);
   const int32_t lf = (int32_t)(f - S->fb);
@@ -302,6 +321,13 @@ __global__ void k_topology_quad(const int32_t* __restrict__ ids, const int32_t* 
   const int32_t fpid = S->nv + lf + off;
   int4 c = reinterpret_cast<const int4*>(ids)[f];
   int4 e = reinterpret_cast<const int4*>(eout)[f];
+  // every corner's edge id was written by the numeric pass; the clamp keeps
+  // refined ids inside the refined vertex range whatever eout holds
+  const int32_t emax = S->ne > 0 ? (int32_t)S->ne - 1 : 0;
+  e.x = min(max(e.x, 0), emax);
+  e.y = min(max(e.y, 0), emax);
+  e.z = min(max(e.z, 0), emax);
+  e.w = min(max(e.w, 0), emax);
   c.x += off - vb;
   c.y += off - vb;
   c.z += off - vb;
@@ -311,6 +337,12 @@ __global__ void k_topology_quad(const int32_t* __restrict__ ids, const int32_t* 
   o[1] = make_int4(c.y, eb + e.y, fpid, eb + e.x);
   o[2] = make_int4(c.z, eb + e.z, fpid, eb + e.y);
   o[3] = make_int4(c.w, eb + e.w, fpid, eb + e.z);
+  if (S->next_cb >= 0) {  // face point's row of the next level: corner 2 of its 4 refined quads
+    const int64_t row = S->next_cb + S->nnz + 4 * (int64_t)lf;
+    L_ntptr[S->next_vb + S->nv + lf] = (int32_t)row;
+    const int32_t c0 = (int32_t)(S->next_cb + 16 * (int64_t)lf + 2);
+    *reinterpret_cast<int4*>(L_ntids + row) = make_int4(c0, c0 + 4, c0 + 8, c0 + 12);
+  }
   if (S->dst_ptr) {  // createQuadColPtr: ptr[t] = 4t
     int32_t b = 16 * lf;
     reinterpret_cast<int4*>(S->dst_ptr)[lf] = make_int4(b, b + 4, b + 8, b + 12);
@@ -327,8 +359,9 @@ __global__ void k_topology_quad(const int32_t* __restrict__ ids, const int32_t* 
 }
 
 __global__ void k_topology_poly(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr,
-                                const int32_t* __restrict__ eout, int32_t nf, SegTable seg) {
-  int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+                                const int32_t* __restrict__ eout, int32_t nf, SegTable seg,
+                                int32_t* __restrict__ L_ntptr, int32_t* __restrict__ L_ntids) {
+  int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (f >= nf) return;
   const SegDesc* S = seg.d + seg_of(seg.fbase, seg.n, f);
   const int32_t lf = (int32_t)(f - S->fb);
@@ -339,13 +372,17 @@ __global__ void k_topology_poly(const int32_t* __restrict__ ids, const int32_t* 
   const int64_t cb = S->cb;
   int32_t s = fptr[f], e = fptr[f + 1];
   int4* o = reinterpret_cast<int4*>(S->dst_ids);
+  const int32_t emax = S->ne > 0 ? (int32_t)S->ne - 1 : 0;
   for (int32_t t = s; t < e; ++t) {
     int32_t tp = (t == s) ? e - 1 : t - 1;
     int64_t lt = t - cb;
-    o[lt] = make_int4(ids[t] - vb + off, eb + eout[t], fpid, eb + eout[tp]);
+    int32_t e0 = min(max(eout[t], 0), emax), e1 = min(max(eout[tp], 0), emax);
+    o[lt] = make_int4(ids[t] - vb + off, eb + e0, fpid, eb + e1);
     if (S->dst_ptr) S->dst_ptr[lt] = (int32_t)(4 * lt);
     if (S->dst_vals) reinterpret_cast<int4*>(S->dst_vals)[lt] = make_int4(1, 2, 3, 4);
+    if (S->next_cb >= 0) L_ntids[S->next_cb + S->nnz + lt] = (int32_t)(S->next_cb + 4 * lt + 2);
   }
+  if (S->next_cb >= 0) L_ntptr[S->next_vb + S->nv + lf] = (int32_t)(S->next_cb + S->nnz + (s - cb));
   if (S->dst_ptr && lf == S->nf - 1) S->dst_ptr[S->nnz] = (int32_t)(4 * S->nnz);
 }
 
@@ -429,6 +466,15 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(const int32_t* __restrict_
 }
 
 }  // namespace
+
+void init_device_options() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    const char* e = std::getenv("SLAK_XCD_REMAP");
+    int on = e ? std::atoi(e) : 1;
+    (void)hipMemcpyToSymbol(HIP_SYMBOL(g_xcd_remap), &on, sizeof(int));
+  });
+}
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -538,12 +584,12 @@ void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st) {
 }
 
 void launch_topology(bool quad, const int32_t* ids, const int32_t* fptr, const int32_t* eout, int32_t nf,
-                     const SegTable& seg, hipStream_t st) {
+                     const SegTable& seg, int32_t* ntptr, int32_t* ntids, hipStream_t st) {
   if (nf <= 0) return;
   if (quad)
-    k_topology_quad<<<grid_for(nf), kBlock, 0, st>>>(ids, eout, nf, seg);
+    k_topology_quad<<<grid_for(nf), kBlock, 0, st>>>(ids, eout, nf, seg, ntptr, ntids);
   else
-    k_topology_poly<<<grid_for(nf), kBlock, 0, st>>>(ids, fptr, eout, nf, seg);
+    k_topology_poly<<<grid_for(nf), kBlock, 0, st>>>(ids, fptr, eout, nf, seg, ntptr, ntids);
 }
 
 int64_t scan_partials_len(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }

@@ -72,11 +72,18 @@ int autotune_cap(bool quad, int64_t nv, int64_t long_rows, int max_row) {
 }
 
 Plan::Plan(int scheme, std::vector<MeshJob> jobs, OutputMode mode, hipStream_t st)
-    : scheme_(scheme), jobs_(std::move(jobs)), mode_(mode), st_(st), arena_(pool()) {}
+    : scheme_(scheme), jobs_(std::move(jobs)), mode_(mode), st_(st), arena_(pool()) {
+  init_device_options();
+}
 
 Plan::~Plan() {
   if (exec_) (void)hipGraphExecDestroy(exec_);
   if (graph_) (void)hipGraphDestroy(graph_);
+  if (owned_st_) {
+    (void)hipStreamSynchronize(owned_st_);
+    (void)hipStreamDestroy(owned_st_);
+    (void)hipEventDestroy(order_ev_);
+  }
   bool held = false;
   for (auto& j : jobs_) held |= j.out_ids != nullptr;
   if (!outputs_released_ && held) {
@@ -190,6 +197,8 @@ void Plan::plan_first_level() {
   L.ecnt = arena_.get_n<int32_t>(L.nv);
   L.eptr = arena_.get_n<int32_t>(L.nv + 1);
   L.eout = arena_.get_n<int32_t>(L.nnz);
+  L.bcnt = arena_.get_n<int32_t>(L.nv);
+  L.bptr = arena_.get_n<int32_t>(L.nv + 1);
 }
 
 void Plan::record_analysis(LevelPlan& L, bool count) {
@@ -202,11 +211,11 @@ void Plan::record_analysis(LevelPlan& L, bool count) {
     ProfScope ps(st_, L.quad ? "count_quad" : "count_poly", 4.0 * L.nnz + 8.0 * L.nv + (L.quad ? 0.0 : 4.0 * L.nf));
     launch_faces(L.quad, false, true, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), L.cnt, st_);
   }
-  {
-    ProfScope ps(st_, "scan", 8.0 * L.nv + 4);
-    launch_scan(L.cnt, L.tptr, L.nv, d_partial_, d_flags_, st_);
-  }
-  {
+  if (count) {  // M^T by counting sort; later levels receive it from the level before
+    {
+      ProfScope ps(st_, "scan", 8.0 * L.nv + 4);
+      launch_scan(L.cnt, L.tptr, L.nv, d_partial_, d_flags_, st_);
+    }
     ProfScope ps(st_, L.quad ? "transpose_quad" : "transpose_poly",
                  8.0 * L.nnz + 12.0 * L.nv + (L.quad ? 0.0 : 4.0 * L.nf));
     launch_transpose(L.quad, L.ids, L.fptr, (int32_t)L.nf, L.tptr, L.cnt, L.tids, st_);
@@ -222,15 +231,21 @@ void Plan::record_analysis(LevelPlan& L, bool count) {
   V.tptr = L.tptr;
   V.tids = L.tids;
   V.ecnt = L.ecnt;
+  V.bcnt = L.bcnt;
   V.flags = d_flags_;
   V.seg = L.table();
   {
-    ProfScope ps(st_, L.quad ? "spgemm_symbolic_quad" : "spgemm_symbolic_poly", 8.0 * L.nnz + 8.0 * L.nv + poly_b);
+    ProfScope ps(st_, L.quad ? "spgemm_symbolic_quad" : "spgemm_symbolic_poly",
+                 8.0 * L.nnz + 8.0 * L.nv + (L.bcnt ? 4.0 * L.nv : 0.0) + poly_b);
     launch_symbolic(L.quad, scheme_, L.cap, V, st_);
   }
   {
     ProfScope ps(st_, "scan", 8.0 * L.nv + 4);
     launch_scan(L.ecnt, L.eptr, L.nv, d_partial_, d_flags_, st_);
+  }
+  if (L.bcnt) {
+    ProfScope ps(st_, "scan", 8.0 * L.nv + 4);
+    launch_scan(L.bcnt, L.bptr, L.nv, d_partial_, d_flags_, st_);
   }
 }
 
@@ -458,30 +473,38 @@ void Plan::prepare() {
         S.dst_ptr = J.out_ptr;
         S.dst_vals = J.out_vals;
         S.id_offset = 0;
+        S.next_vb = -1;
+        S.next_cb = -1;
       } else {
         S.dst_verts = Un_P + vbn;
         S.dst_ids = Un_ids + cbn;
         S.dst_ptr = nullptr;
         S.dst_vals = nullptr;
         S.id_offset = vbn;
+        S.next_vb = vbn;
+        S.next_cb = cbn;
         vbn += J.nv[l];
         cbn += J.nnz[l];
       }
     }
+    L.next_nv = nv_next;
+    L.next_nnz = nnz_next;
     if (l < Lmax) {
       LevelPlan& nx = levels_[l];
       nx.ids = Un_ids;
       nx.P = Un_P;
     }
   }
-  // scratch shared by levels >= 2
+  // Levels >= 2 receive their M^T from the level before (two alternating
+  // buffers) and share the rest of the scratch, sized for the largest level.
   if (Lmax >= 2) {
-    int32_t* cnt = arena_.get_n<int32_t>(max_nv);
-    int32_t* tptr = arena_.get_n<int32_t>(max_nv + 1);
-    int32_t* tids = arena_.get_n<int32_t>(max_nnz);
+    int32_t* mt_ptr[2] = {arena_.get_n<int32_t>(max_nv + 1), Lmax >= 3 ? arena_.get_n<int32_t>(max_nv + 1) : nullptr};
+    int32_t* mt_ids[2] = {arena_.get_n<int32_t>(max_nnz), Lmax >= 3 ? arena_.get_n<int32_t>(max_nnz) : nullptr};
     int32_t* ecnt = arena_.get_n<int32_t>(max_nv);
     int32_t* eptr = arena_.get_n<int32_t>(max_nv + 1);
     int32_t* eout = arena_.get_n<int32_t>(max_nnz);
+    int32_t* bcnt = nullptr;
+    int32_t* bptr = nullptr;
     int64_t need2 = scan_partials_len(max_nv);
     if (need2 > partial_len_) {
       d_partial_ = arena_.get_n<int64_t>(need2);
@@ -489,12 +512,26 @@ void Plan::prepare() {
     }
     for (int l = 2; l <= Lmax; ++l) {
       LevelPlan& L = levels_[l - 1];
-      L.cnt = cnt;
-      L.tptr = tptr;
-      L.tids = tids;
+      L.tptr = mt_ptr[l % 2];
+      L.tids = mt_ids[l % 2];
       L.ecnt = ecnt;
       L.eptr = eptr;
       L.eout = eout;
+      levels_[l - 2].ntptr = L.tptr;
+      levels_[l - 2].ntids = L.tids;
+      bool boundary = false;
+      for (int j : L.jobs) {
+        const MeshJob& J = jobs_[j];
+        boundary |= 2 * J.ne[l - 1] != J.nnz[l - 1];  // n_ext = 2 n_e - nnz
+      }
+      if (boundary) {
+        if (!bcnt) {
+          bcnt = arena_.get_n<int32_t>(max_nv);
+          bptr = arena_.get_n<int32_t>(max_nv + 1);
+        }
+        L.bcnt = bcnt;
+        L.bptr = bptr;
+      }
     }
   }
   for (int l = 1; l <= Lmax; ++l) {
@@ -516,27 +553,12 @@ void Plan::record_level(int l) {
   const double poly_b = L.quad ? 0.0 : 4.0 * (L.nf + 1) + 4.0 * L.nnz;
   double ne_tot = 0;
   for (auto& S : L.segs) ne_tot += (double)S.ne;
-  if (l > 1) {
-    {
-      ProfScope ps(st_, "fill", 4.0 * L.nv);
-      launch_fill_i32(L.cnt, L.nv, 0, st_);
-    }
-    {
-      ProfScope ps(st_, "facepoints+count", 4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 8.0 * L.nv);
-      launch_faces(true, true, true, L.ids, nullptr, L.P, (int32_t)L.nf, L.table(), L.cnt, st_);
-    }
-    record_analysis(L, false);
-  } else {
+  {
     ProfScope ps(st_, L.quad ? "facepoints_quad" : "facepoints_poly",
                  4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + (L.quad ? 0.0 : 4.0 * (L.nf + 1)));
     launch_faces(L.quad, true, false, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), nullptr, st_);
   }
-  {
-    // every corner's edge id is written by the numeric pass for a valid mesh;
-    // zero-filling keeps refined ids in range even if one were not
-    ProfScope ps(st_, "fill", 4.0 * L.nnz);
-    launch_fill_i32(L.eout, L.nnz, 0, st_);
-  }
+  if (l > 1) record_analysis(L, false);  // M^T came from the level before
   LevelView V{};
   V.ids = L.ids;
   V.fptr = L.fptr;
@@ -549,13 +571,20 @@ void Plan::record_level(int l) {
   V.tids = L.tids;
   V.eptr = L.eptr;
   V.eout = L.eout;
+  V.bptr = L.bptr;
+  V.ntptr = L.ntptr;
+  V.ntids = L.ntids;
   V.flags = d_flags_;
   V.seg = L.table();
+  if (L.ntptr) {  // last offset of the next level's M^T
+    launch_fill_i32(L.ntptr + L.next_nv, 1, (int32_t)L.next_nnz, st_);
+  }
   {
-    // reads tptr, tids, ids, eptr, P, face points; writes vertex points, edge points, eout
+    // reads tptr, tids, ids, eptr, P, face points; writes vertex points, edge
+    // points, eout and the vertex and edge rows of the next level's M^T
     ProfScope ps(st_, L.quad ? "spgemm_numeric_quad" : "spgemm_numeric_poly",
                  8.0 * L.nv + 8.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 16.0 * L.nv + 16.0 * ne_tot + 4.0 * L.nnz +
-                     4.0 * L.nv + poly_b);
+                     4.0 * L.nv + poly_b + (L.ntptr ? 4.0 * (L.nv + ne_tot) + 12.0 * L.nnz : 0.0));
     launch_numeric(L.quad, scheme_, L.cap, V, st_);
   }
   {
@@ -567,8 +596,9 @@ void Plan::record_level(int l) {
         nnz_final += (double)L.segs[k].nnz;
       }
     ProfScope ps(st_, L.quad ? "topology_quad" : "topology_poly",
-                 8.0 * L.nnz + 16.0 * L.nnz + (lak_final ? 20.0 * nnz_final : 0.0) + (L.quad ? 0.0 : 4.0 * L.nf));
-    launch_topology(L.quad, L.ids, L.fptr, L.eout, (int32_t)L.nf, L.table(), st_);
+                 8.0 * L.nnz + 16.0 * L.nnz + (lak_final ? 20.0 * nnz_final : 0.0) + (L.quad ? 0.0 : 4.0 * L.nf) +
+                     (L.ntptr ? 4.0 * L.nf + 4.0 * L.nnz : 0.0));
+    launch_topology(L.quad, L.ids, L.fptr, L.eout, (int32_t)L.nf, L.table(), L.ntptr, L.ntids, st_);
   }
 }
 
@@ -594,6 +624,28 @@ void Plan::launch(bool include_analysis) {
 
 void Plan::run_graph() {
   if (levels_.empty()) return;
+  // The legacy default stream cannot be captured: graph plans run on a stream
+  // of their own, ordered after the caller's default-stream work by an event.
+  if (!capture_st_) {
+    if (st_) {
+      capture_st_ = st_;
+    } else {
+      SLAK_HIP_CHECK(hipStreamCreateWithFlags(&owned_st_, hipStreamNonBlocking));
+      SLAK_HIP_CHECK(hipEventCreateWithFlags(&order_ev_, hipEventDisableTiming));
+      capture_st_ = owned_st_;
+    }
+  }
+  if (owned_st_) {
+    SLAK_HIP_CHECK(hipEventRecord(order_ev_, nullptr));
+    SLAK_HIP_CHECK(hipStreamWaitEvent(owned_st_, order_ev_, 0));
+  }
+  hipStream_t saved = st_;
+  st_ = capture_st_;
+  struct Restore {
+    hipStream_t& ref;
+    hipStream_t val;
+    ~Restore() { ref = val; }
+  } restore{st_, saved};
   if (!exec_) {
     // event-pair timing stays out of the captured graph
     const bool prof = Profiler::get().enabled();
@@ -613,6 +665,11 @@ void Plan::run_graph() {
     SLAK_HIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
   }
   SLAK_HIP_CHECK(hipGraphLaunch(exec_, st_));
+  if (owned_st_) {
+    // later work on the caller's default stream sees the results
+    SLAK_HIP_CHECK(hipEventRecord(order_ev_, owned_st_));
+    SLAK_HIP_CHECK(hipStreamWaitEvent(nullptr, order_ev_, 0));
+  }
 }
 
 void Plan::finish() {

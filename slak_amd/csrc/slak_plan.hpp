@@ -48,6 +48,10 @@ struct LevelPlan {
   int max_row = -1;
   // scratch (the first level owns its own; later levels share the largest)
   int32_t *cnt = nullptr, *tptr = nullptr, *tids = nullptr, *ecnt = nullptr, *eptr = nullptr, *eout = nullptr;
+  int32_t *bcnt = nullptr, *bptr = nullptr;  // owned boundary edges (levels that may have a boundary)
+  // the next level's M^T, written by this level's numeric and topology kernels
+  int32_t *ntptr = nullptr, *ntids = nullptr;
+  int64_t next_nv = 0, next_nnz = 0;  // next level's union sizes
   std::vector<int> jobs;  // segment -> job
   std::vector<SegDesc> segs;
   std::vector<int64_t> vbase, fbase;
@@ -117,6 +121,9 @@ class Plan {
   int syncs_ = 0;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
+  hipStream_t capture_st_ = nullptr;  // stream the graph is captured on and replayed
+  hipStream_t owned_st_ = nullptr;    // created when the caller's stream is the default one
+  hipEvent_t order_ev_ = nullptr;
   std::vector<void*> pooled_outputs_;
 };
 
