@@ -68,6 +68,13 @@ def build(verbose: bool = True) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"drop-in test build failed:\n{r.stderr}")
+    # command-line driver with the reference demo's interface
+    cli = os.path.join(LIBDIR, "slak_cli")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "tools", "slak_cli.cpp"), "-L", LIBDIR, "-lslak", "-Wl,-rpath,$ORIGIN", "-o", cli]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"slak_cli build failed:\n{r.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB
