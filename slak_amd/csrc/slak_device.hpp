@@ -1,18 +1,23 @@
 // Device-side building blocks of the Catmull-Clark level pipeline (gfx950).
 //
 // A level runs over a union of meshes ("segments"): mesh k owns the vertex
-// range [vb_k, vb_k + nv_k), faces [fb_k, ...) and corners [cb_k, ...) of the
-// level's arrays, which hold global (union) indices. Data layout in HBM:
+// range [vb_k, vb_k + nv_k), faces [fb_k, ...), corners [cb_k, ...) and edges
+// [eb_k, eb_k + ne_k) of the level's arrays, which hold global (union)
+// indices. Data layout in HBM:
 //   ids[nnz]           position-ordered corner -> vertex (quad levels: 4 per face, int4-aligned)
 //   fptr[nf+1], cface[nnz]   face offsets and corner -> face (poly levels only)
 //   P[nv]              float4 positions (x, y, z, w=1)
-//   tptr[nv+1], tids[nnz]    M^T: vertex -> incident corners (corner index t)
-//   eptr[nv+1]         per-vertex offset of the edges the vertex owns (global scan)
-//   eout[nnz]          mesh-local edge id of every corner's outgoing half-edge
+//   tptr[nv+1], trow[nnz]    M^T: vertex -> incident corners, ascending, each
+//                      with its in-neighbour (the corner's predecessor vertex)
+//   ci[nnz]            per corner: mesh-local edge id of the outgoing
+//                      half-edge and the twin corner (-1 on the boundary)
+//   bnd[ne], blk[ne+1] boundary flag per edge and its exclusive scan (levels
+//                      with a boundary only)
 // Outputs go to each segment's own destination: refined positions
 // [vertex points | face points | edge points] and refined quads in the
 // mesh's local numbering (+ the segment's id offset when the result feeds the
-// next level's union).
+// next level's union). A level that feeds another also writes that level's
+// tptr/trow/ci/bnd in closed form, so only the first level numbers edges.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -40,6 +45,7 @@ struct SegDesc {
   int32_t nv, nf;          // local sizes
   int64_t nnz;             // local corners
   int64_t ne;              // local edges (edge points the destination holds)
+  int64_t eb;              // union base of the mesh's edges (boundary flags)
   float4* dst_verts;       // nv + nf + ne refined positions
   int32_t* dst_ids;        // 4 * nnz refined quad ids
   int32_t* dst_ptr;        // LAK form (final level) or null
@@ -47,7 +53,7 @@ struct SegDesc {
   int64_t id_offset;       // added to local refined ids
   // next level's union bases of this mesh (-1: the mesh finishes here); the
   // level writes the refined mesh's M^T rows there directly
-  int64_t next_vb, next_cb;
+  int64_t next_vb, next_cb, next_eb;
   // first-level sources (the caller's mesh)
   const int32_t* src_ptr;  // null: homogeneous with src_stride
   const int32_t* src_ids;
@@ -88,19 +94,16 @@ struct LevelView {
   int32_t nf;
   int64_t nnz;
   const int32_t* tptr;
-  int32_t* tids;
+  int32_t* tids;         // M^T corners (first level: transposed, rows sorted on use)
+  int2* trow;            // sorted rows with in-neighbours (written by the symbolic pass), or null
   int32_t* ecnt;
   const int32_t* eptr;
-  int32_t* eout;
-  int32_t* E_ids;   // optional LAK edge matrix rows (single segment)
-  int32_t* E_vals;  // optional LAK edge matrix values (+-(id+1))
+  int32_t* ci;           // (edge id, twin) pairs per corner, written by the number pass
+  int32_t* bnd;          // union edge boundary flags (number pass), or null: closed level
+  int32_t* E_ids;   // LAK edge matrix rows (SpGEMM operator)
+  int32_t* E_vals;  // LAK edge matrix values (+-(id+1))
   int* flags;
   SegTable seg;
-  // refined incidence for the next level (null when no mesh continues)
-  int32_t* ntptr;
-  int32_t* ntids;
-  int32_t* bcnt;         // owned boundary edges per vertex (symbolic), or null
-  const int32_t* bptr;   // its exclusive scan (numeric), or null: no boundary
   // mapped SpGEMM operator outputs (F, D rows share M^T's row offsets)
   int32_t* F_ids;
   int32_t* F_vals;
@@ -354,88 +357,25 @@ __device__ __forceinline__ void row_symbolic(const Row& r, int32_t v, int32_t& c
   bcount = b;
 }
 
-// Numeric pass: edge ids, edge points and the vertex point of row v, plus the
-// edge id of every half-edge of the edges v owns (eout).
-template <int SCHEME, bool QUADT, class Row>
-__device__ __forceinline__ void row_numeric(Row& r, const LevelView& L, int32_t v) {
+// Number pass of the first level: the mesh-local id of every edge v owns
+// (LAK: column v of E, spgemm.cuh:71-100; SLAK: flagged out-half-edges in
+// corner order, CatmullClarkKernels.cuh:147-236) stored at both of its
+// half-edges' corners, the twin corner of each out-half-edge of v, and the
+// boundary flag of every owned edge by union edge index.
+template <int SCHEME, class Row>
+__device__ __forceinline__ void row_number(Row& r, const LevelView& L, int32_t v) {
   constexpr int C = Row::kCap;
-  // local copies: lambdas below capture locals only, so the kernel argument
-  // block never has its address taken (it would be spilled to scratch)
   const int k = seg_of(L.seg.vbase, L.seg.n, v);
   const SegDesc* __restrict__ S = L.seg.d + k;
-  const int64_t vb = S->vb, fb = S->fb;
-  const int32_t snv = S->nv, snf = S->nf;
   const int64_t ne_cap = S->ne;
-  float4* __restrict__ dst = S->dst_verts;
-  const float4* __restrict__ P = L.P;
-  auto FP = [dst, snv, fb](int32_t f) __attribute__((always_inline)) { return dst[snv + (f - fb)]; };
-  float4* __restrict__ ep = dst + snv + snf;
-  int32_t* __restrict__ eout = L.eout;
-  int32_t* __restrict__ E_ids = L.E_ids;
-  int32_t* __restrict__ E_vals = L.E_vals;
+  const int64_t eb = S->eb;
+  int32_t* __restrict__ ci = L.ci;
+  int32_t* __restrict__ bnd = L.bnd;
   int* flags = L.flags;
-  const float4 Pv = P[v];
-  const int32_t e0 = L.eptr[v] - L.eptr[vb];  // mesh-local edge base of v
+  const int32_t e0 = L.eptr[v] - L.eptr[S->vb];  // mesh-local edge base of v
   const int n = r.n;
-  // refined incidence of the next level: local corner c of this level maps to
-  // the refined quads 4c..4c+3, i.e. next-level corners ncb + 4c + j
-  const int64_t ncb = S->next_cb, nvb = S->next_vb, cb = S->cb;
-  const int64_t snnz = S->nnz;
-  int32_t* __restrict__ ntptr = L.ntptr;
-  int32_t* __restrict__ ntids = L.ntids;
-  const int32_t* __restrict__ cface = L.cface;
-  const int32_t* __restrict__ fptr = L.fptr;
-  const int32_t bb = L.bptr ? L.bptr[v] - L.bptr[vb] : 0;  // boundary edges owned by earlier vertices
-  auto next_corner = [cface, fptr](int32_t t) __attribute__((always_inline)) {
-    if (QUADT) {
-      return (t & ~3) | ((t + 1) & 3);
-    } else {
-      int32_t f = cface[t];
-      return (t + 1 == fptr[f + 1]) ? fptr[f] : t + 1;
-    }
-  };
-  // row of edge point e: corners (4 ta + 1, 4 next(ta) + 3[, 4 tb + 1, 4 next(tb) + 3]), sorted
-  auto edge_row = [&](int32_t e, int32_t bl, int32_t ta, int32_t tb) __attribute__((always_inline)) {
-    if (ncb < 0 || e >= ne_cap) return;
-    const int64_t row = ncb + 2 * snnz + 4 * (int64_t)e - 2 * (int64_t)bl;
-    ntptr[nvb + snv + snf + e] = (int32_t)row;
-    int32_t a0 = 4 * (int32_t)(ta - cb) + 1, a1 = 4 * (int32_t)(next_corner(ta) - cb) + 3;
-    if (a1 < a0) {
-      int32_t x = a0;
-      a0 = a1;
-      a1 = x;
-    }
-    if (tb < 0) {
-      ntids[row] = (int32_t)(ncb + a0);
-      ntids[row + 1] = (int32_t)(ncb + a1);
-      return;
-    }
-    int32_t b0 = 4 * (int32_t)(tb - cb) + 1, b1 = 4 * (int32_t)(next_corner(tb) - cb) + 3;
-    if (b1 < b0) {
-      int32_t x = b0;
-      b0 = b1;
-      b1 = x;
-    }
-    // merge two sorted pairs
-    int32_t m0 = min(a0, b0), m3 = max(a1, b1);
-    int32_t lo = max(a0, b0), hi = min(a1, b1);
-    int32_t m1 = min(lo, hi), m2 = max(lo, hi);
-    ntids[row] = (int32_t)(ncb + m0);
-    ntids[row + 1] = (int32_t)(ncb + m1);
-    ntids[row + 2] = (int32_t)(ncb + m2);
-    ntids[row + 3] = (int32_t)(ncb + m3);
-  };
-  // LAK: boundary edges among v's owned edges with a smaller neighbour than x
-  auto lak_brank = [&](int32_t x) __attribute__((always_inline)) {
-    int k = 0;
-    rloop<C>(n, [&](int j) __attribute__((always_inline)) {
-      k += (r.W(j) < x) && (r.JM(j) < 0);
-      k += (r.U(j) < x) && (r.IM(j) < 0);
-    });
-    return k;
-  };
-  // Edge ids beyond the segment's edge points are clamped (and flagged) so
-  // that refined ids stay inside the refined vertex range.
+  // Edge ids beyond the segment's edges are clamped (and flagged) so that
+  // every later index stays inside the mesh's ranges.
   auto cap = [flags, ne_cap](int32_t e) __attribute__((always_inline)) {
     if (e >= ne_cap) {
       atomicOr(flags, kFlagOverflow);
@@ -443,88 +383,72 @@ __device__ __forceinline__ void row_numeric(Row& r, const LevelView& L, int32_t 
     }
     return e;
   };
-
+  auto put = [ci, bnd, eb](int32_t t, int32_t tw, int32_t e, bool boundary) __attribute__((always_inline)) {
+    ci[2 * (int64_t)t] = e;
+    if (tw >= 0) ci[2 * (int64_t)tw] = e;
+    if (bnd) bnd[eb + e] = boundary ? 1 : 0;
+  };
   if (SCHEME == kLAK) {
     r.rank_lower(v);
     rloop<C>(n, [&](int i) __attribute__((always_inline)) {
-      int32_t x = r.W(i);
-      if (x < v) {
-        int32_t e = cap(e0 + r.RKO(i));
-        int jm = r.JM(i);
-        float4 q;
-        if (jm >= 0)  // interior: f0 = face of x->v, f1 = face of v->x
-          q = f4mul(f4add(f4add(f4add(P[x], Pv), FP(r.FJ(i))), FP(r.F(i))), 0.25f);
-        else
-          q = f4mul(f4add(P[x], Pv), 0.5f);
-        if (e < ne_cap) ep[e] = q;
-        eout[r.T(i)] = e;
-        if (jm >= 0) eout[r.TPJ(i)] = e;
-        if (E_ids && e < ne_cap) {
-          E_ids[e] = x;
-          E_vals[e] = jm >= 0 ? -(e + 1) : (e + 1);
-        }
-        if (ncb >= 0) edge_row(e, bb + (L.bptr ? lak_brank(x) : 0), r.T(i), jm >= 0 ? r.TPJ(i) : -1);
+      if (r.W(i) < v) {
+        const int jm = r.JM(i);
+        put(r.T(i), jm >= 0 ? r.TPJ(i) : -1, cap(e0 + r.RKO(i)), jm < 0);
       }
-      int32_t y = r.U(i);
-      if (y < v && r.IM(i) < 0) {  // boundary edge seen only as y -> v
-        int32_t e = cap(e0 + r.RKI(i));
-        if (e < ne_cap) ep[e] = f4mul(f4add(P[y], Pv), 0.5f);
-        eout[r.TP(i)] = e;
-        if (E_ids && e < ne_cap) {
-          E_ids[e] = y;
-          E_vals[e] = e + 1;
-        }
-        if (ncb >= 0) edge_row(e, bb + (L.bptr ? lak_brank(y) : 0), r.TP(i), -1);
-      }
+      if (r.U(i) < v && r.IM(i) < 0)  // boundary edge seen only as u -> v
+        put(r.TP(i), -1, cap(e0 + r.RKI(i)), true);
     });
   } else {
-    int kk = 0, kb = 0;
+    int kk = 0;
     rloop<C>(n, [&](int i) __attribute__((always_inline)) {
-      int32_t x = r.W(i);
-      int jm = r.JM(i);
-      if (jm < 0 || x > v) {
-        int32_t e = cap(e0 + kk);
+      const int jm = r.JM(i);
+      if (jm < 0 || r.W(i) > v) {
+        put(r.T(i), jm >= 0 ? r.TPJ(i) : -1, cap(e0 + kk), jm < 0);
         ++kk;
-        if (ncb >= 0) edge_row(e, bb + kb, r.T(i), jm >= 0 ? r.TPJ(i) : -1);
-        kb += jm < 0;
-        float4 q;
-        if (jm >= 0)  // interior, v < x: f0 = face of v->x, f1 = face of x->v
-          q = f4mul(f4add(f4add(f4add(Pv, P[x]), FP(r.F(i))), FP(r.FJ(i))), 0.25f);
-        else
-          q = (v < x) ? f4mul(f4add(Pv, P[x]), 0.5f) : f4mul(f4add(P[x], Pv), 0.5f);
-        if (e < ne_cap) ep[e] = q;
-        eout[r.T(i)] = e;
-        if (jm >= 0) eout[r.TPJ(i)] = e;
       }
     });
   }
+  rloop<C>(n, [&](int i) __attribute__((always_inline)) {
+    ci[2 * (int64_t)r.T(i) + 1] = r.JM(i) >= 0 ? r.TPJ(i) : -1;
+  });
+}
 
-  // vertex point (LAKCatmullClarkImpl.cu:200-235, pull form)
-  bool bnd = false;
-  rloop<C>(n, [&](int i) __attribute__((always_inline)) { bnd |= (r.JM(i) < 0) || (r.IM(i) < 0); });
-  float4 q;
-  if (!bnd) {
-    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-    rloop<C>(n, [&](int i) __attribute__((always_inline)) { s = f4add(s, FP(r.F(i))); });
-    rloop<C>(n, [&](int j) __attribute__((always_inline)) { s = f4add(s, P[r.U(j)]); });
-    float nf = (float)n;
-    q = f4div(f4add(f4div(s, nf), f4mul(Pv, nf - 2.0f)), nf);
+// One level's refinement inputs and the next level's structure.
+struct RefineView {
+  const int32_t* ids;
+  const int32_t* fptr;   // poly levels
+  const int32_t* cface;  // poly levels
+  const float4* P;
+  int32_t nv;
+  int32_t nf;
+  int64_t nnz;
+  const int32_t* tptr;
+  const int2* trow;      // (corner, in-neighbour), sorted by corner
+  const int2* ci;        // (mesh-local edge id of the outgoing half-edge, twin corner or -1)
+  const int32_t* blk;    // exclusive scan of the union edge boundary flags, or null: closed level
+  SegTable seg;
+  // next level (null when no mesh continues)
+  int32_t* ntptr;
+  int2* ntrow;
+  int2* nci;
+  int32_t* nbnd;         // next level's union edge boundary flags, or null: closed
+};
+
+// Face, successor and predecessor corner of corner t (quad: bit arithmetic;
+// poly: through cface / fptr).
+template <bool QUAD>
+__device__ __forceinline__ void corner_nbrs(const int32_t* __restrict__ fptr, const int32_t* __restrict__ cface,
+                                            int32_t t, int32_t& f, int32_t& tn, int32_t& tp) {
+  if (QUAD) {
+    f = t >> 2;
+    const int32_t b = t & ~3;
+    tn = b | ((t + 1) & 3);
+    tp = b | ((t + 3) & 3);
   } else {
-    q = f4mul(Pv, 0.75f);
-    rloop<C>(n, [&](int i) __attribute__((always_inline)) {
-      if (r.JM(i) < 0) q = f4add(q, f4mul(P[r.W(i)], 0.125f));
-    });
-    rloop<C>(n, [&](int j) __attribute__((always_inline)) {
-      if (r.IM(j) < 0) q = f4add(q, f4mul(P[r.U(j)], 0.125f));
-    });
-  }
-  dst[v - vb] = q;
-  if (ncb >= 0) {  // row of the vertex itself: corner 0 of each refined quad at v
-    const int64_t row = ncb + (L.tptr[v] - cb);
-    ntptr[nvb + (v - vb)] = (int32_t)row;
-    rloop<C>(n, [&](int i) __attribute__((always_inline)) {
-      ntids[row + i] = (int32_t)(ncb + 4 * (r.T(i) - cb));
-    });
+    f = cface[t];
+    const int32_t s = fptr[f], e = fptr[f + 1];
+    tn = (t + 1 == e) ? s : t + 1;
+    tp = (t == s) ? e - 1 : t - 1;
   }
 }
 

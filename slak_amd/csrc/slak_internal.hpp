@@ -51,10 +51,10 @@ void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int3
 void launch_transpose(bool quad, const int32_t* ids, const int32_t* fptr, int32_t nf, const int32_t* tptr,
                       int32_t* cnt, int32_t* tids, hipStream_t st);
 void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
-void launch_numeric(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
+void launch_number(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
 void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st);
-void launch_topology(bool quad, const int32_t* ids, const int32_t* fptr, const int32_t* eout, int32_t nf,
-                     const SegTable& seg, int32_t* ntptr, int32_t* ntids, hipStream_t st);
+void launch_refine(bool quad, int scheme, const RefineView& R, hipStream_t st);
+void launch_vertex(bool quad, int scheme, const RefineView& R, hipStream_t st);
 int64_t scan_partials_len(int64_t n);
 void launch_scan(const int32_t* in, int32_t* out, int64_t n, int64_t* partial, int* flags, hipStream_t st);
 

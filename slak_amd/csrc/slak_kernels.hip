@@ -1,17 +1,26 @@
 // HIP kernels of the Catmull-Clark level pipeline for gfx950 (MI355X).
 //
-// One level is:
-//   validate      (first level only) check the input and bring corners into
-//                 position order (sortKeyValuePairsSegmentedInPlace,
-//                 LAKCatmullClarkKernels.cuh:147-180)
+// First level (the caller's mesh):
+//   validate      check the input and bring corners into position order
+//                 (sortKeyValuePairsSegmentedInPlace, LAKCatmullClarkKernels.cuh:147-180)
 //   count         valence per vertex (spmv_right with ones, LAKCatmullClarkImpl.cu:347-356)
 //   scan          exclusive scan (cub_wrappers.cuh:30-43)
 //   transpose     M -> M^T by counting sort (csr2cscEx2, LAKCatmullClarkImpl.cu:381-423)
+//   symbolic      row sizing of the mapped SpGEMM -> edges owned per vertex;
+//                 rows sorted and stored with their in-neighbours
+//   number        mapped SpGEMM rows -> E numbering: edge id and twin of
+//                 every corner's half-edge, boundary flags
+// Every level:
 //   face points   left mapped SpMV fp = M^T P / order (spmv.cuh:64-81)
-//   symbolic      row sizing of the mapped SpGEMM -> edges owned per vertex
-//   numeric       mapped SpGEMM rows with fused epilogue: E numbering, edge
-//                 points and the two right SpMVs of the vertex update
-//   topology      refined quads (refineTopology[Homogeneous], LAKCatmullClarkKernels.cuh:320-431)
+//   refine        per corner: edge points (calculateInternal/ExternalEdgepoints),
+//                 refined quads (refineTopology[Homogeneous],
+//                 LAKCatmullClarkKernels.cuh:320-431) and, for a level that
+//                 feeds another, the refined mesh's twins, edge ids and the
+//                 face- and edge-point rows of its M^T, all in closed form
+//   vertex        per vertex: the two right SpMVs of the vertex update
+//                 (LAKCatmullClarkImpl.cu:200-235) and boundary rule
+//                 (LAKCatmullClarkKernels.cuh:464-511); the vertex rows of the
+//                 refined M^T
 // All kernels are pull-based and atomic-free except the integer valence
 // count and transpose slots, whose result does not depend on atomic order,
 // so every output is bitwise deterministic.
@@ -33,7 +42,8 @@ constexpr int kRowCap = 8;  // M^T rows up to this length stay in registers
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
 // With the remap, consecutive tiles (which gather from neighbouring parts of
-// the mesh) land on the same XCD. Selected once per process (SLAK_XCD_REMAP).
+// the mesh) land on the same XCD. Selected once per process (SLAK_XCD_REMAP=1;
+// off by default: on the 256x128 torus chains it measured 10-25 % slower).
 __device__ int g_xcd_remap;
 
 __device__ __forceinline__ int64_t block_index() {
@@ -253,17 +263,24 @@ __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
   int n = L.tptr[v + 1] - s;
   int32_t count = 0, bcount = 0;
   int bad = 0;
+  int2* __restrict__ trow = L.trow;
   if (n <= CAP) {
     RegRow<CAP, QUAD> r;
     r.load(L, s, n);
     row_symbolic<SCHEME>(r, v, count, bcount, bad);
+    if (trow)
+#pragma unroll
+      for (int i = 0; i < CAP; ++i)
+        if (i < n) trow[s + i] = make_int2(r.T(i), r.U(i));
   } else {
     sort_row_in_place(L.tids + s, n);
     MemRow<QUAD> r(L, s, n);
     row_symbolic<SCHEME>(r, v, count, bcount, bad);
+    if (trow)
+      for (int i = 0; i < n; ++i) trow[s + i] = make_int2(r.T(i), r.U(i));
   }
   L.ecnt[v] = count;
-  if (L.bcnt) L.bcnt[v] = bcount;
+  (void)bcount;
   if (bad) {
     atomicOr(L.flags, bad);
     if (L.seg.mflags) atomicOr(L.seg.mflags + seg_of(L.seg.vbase, L.seg.n, v), bad);
@@ -271,7 +288,7 @@ __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
 }
 
 template <bool QUAD, int SCHEME, int CAP>
-__global__ void __launch_bounds__(kBlock) k_numeric(LevelView L) {
+__global__ void __launch_bounds__(kBlock) k_number(LevelView L) {
   int32_t v = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (v >= L.nv) return;
   int32_t s = L.tptr[v];
@@ -279,13 +296,13 @@ __global__ void __launch_bounds__(kBlock) k_numeric(LevelView L) {
   if (n <= CAP) {
     RegRow<CAP, QUAD> r;
     r.load(L, s, n);
-    row_numeric<SCHEME, QUAD>(r, L, v);
+    row_number<SCHEME>(r, L, v);
   } else {
-    // the symbolic pass may have held this row in registers (another
-    // capacity); the in-memory path needs it sorted (idempotent)
+    // the symbolic pass sorted every row longer than its own capacity; the
+    // in-memory path needs this one sorted whatever that capacity was (idempotent)
     sort_row_in_place(L.tids + s, n);
     MemRow<QUAD> r(L, s, n);
-    row_numeric<SCHEME, QUAD>(r, L, v);
+    row_number<SCHEME>(r, L, v);
   }
 }
 
@@ -306,84 +323,266 @@ __global__ void __launch_bounds__(kBlock) k_spgemm_fde(LevelView L) {
 }
 
 // ---------------------------------------------------------------------------
-// topology: refined face t = (v, ep(v->next), nv + f, ep(prev->v)) in the
-// mesh's local numbering, plus the segment's id offset
+// refine: one thread per corner t of face f, v -> w the corner's out-half-edge
+// (edge e1, twin s1), u -> v the in-half-edge (edge e0, twin s0).
+//   refined quad t = (v, ep(e1), fp(f), ep(e0))   in the mesh's local numbering
+//   edge point e1  (written by the half-edge min(t, s1), or t on the boundary)
+//                  = (P_lo + P_hi + fp(face of lo->hi) + fp(face of hi->lo)) / 4,
+//                    (P_lo + P_hi) / 2 on the boundary
+// and for a level that feeds another, in closed form:
+//   twins          4t -> 4 next(s1) + 3, 4t+1 -> 4 next(t) + 2,
+//                  4t+2 -> 4 prev(t) + 1, 4t+3 -> 4 s0
+//   LAK edge ids   edge point e owns its edges to the two end points, then to
+//                  the face points, ranked by the smaller id: base(e) = 4e - bl(e)
+//   SLAK edge ids  vertices own one edge per corner (M^T storage order), face
+//                  points one per corner (nnz + corner), boundary edge points
+//                  one (2 nnz + bl(e)); the vertex-owned ones are filled in by
+//                  the vertex pass, which knows the M^T rank
+//   M^T rows       face point f: its corners 4t+2; edge point e: 4t+1,
+//                  4 next(t) + 3 of both half-edges, sorted
+// bl(e) = boundary edges with a smaller id (0 on closed levels).
 
-__global__ void k_topology_quad(const int32_t* __restrict__ ids, const int32_t* __restrict__ eout, int32_t nf,
-                                SegTable seg, int32_t* __restrict__ L_ntptr, int32_t* __restrict__ L_ntids) {
-  int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
-  if (f >= nf) return;
-  const SegDesc* S = seg.d + seg_of(seg.fbase, seg.n, f);
-  const int32_t lf = (int32_t)(f - S->fb);
-  const int32_t vb = (int32_t)S->vb;
-  const int32_t off = (int32_t)S->id_offset;
-  const int32_t eb = S->nv + S->nf + off;
-  const int32_t fpid = S->nv + lf + off;
-  int4 c = reinterpret_cast<const int4*>(ids)[f];
-  int4 e = reinterpret_cast<const int4*>(eout)[f];
-  // every corner's edge id was written by the numeric pass; the clamp keeps
-  // refined ids inside the refined vertex range whatever eout holds
+__device__ __forceinline__ int32_t clampi(int32_t x, int32_t hi) { return min(max(x, 0), hi); }
+
+template <bool QUAD, int SCHEME>
+__global__ void __launch_bounds__(kBlock) k_refine(RefineView R) {
+  const int64_t t64 = block_index() * blockDim.x + threadIdx.x;
+  if (t64 >= R.nnz) return;
+  const int32_t t = (int32_t)t64;
+  int32_t f, tn, tp;
+  corner_nbrs<QUAD>(R.fptr, R.cface, t, f, tn, tp);
+  const SegDesc* __restrict__ S = R.seg.d + seg_of(R.seg.fbase, R.seg.n, f);
+  const int32_t vb = (int32_t)S->vb, fb = (int32_t)S->fb, cb = (int32_t)S->cb;
+  const int32_t snv = S->nv, snf = S->nf;
+  const int32_t snnz = (int32_t)S->nnz;
   const int32_t emax = S->ne > 0 ? (int32_t)S->ne - 1 : 0;
-  e.x = min(max(e.x, 0), emax);
-  e.y = min(max(e.y, 0), emax);
-  e.z = min(max(e.z, 0), emax);
-  e.w = min(max(e.w, 0), emax);
-  c.x += off - vb;
-  c.y += off - vb;
-  c.z += off - vb;
-  c.w += off - vb;
-  int4* o = reinterpret_cast<int4*>(S->dst_ids) + 4 * (int64_t)lf;
-  o[0] = make_int4(c.x, eb + e.x, fpid, eb + e.w);
-  o[1] = make_int4(c.y, eb + e.y, fpid, eb + e.x);
-  o[2] = make_int4(c.z, eb + e.z, fpid, eb + e.y);
-  o[3] = make_int4(c.w, eb + e.w, fpid, eb + e.z);
-  if (S->next_cb >= 0) {  // face point's row of the next level: corner 2 of its 4 refined quads
-    const int64_t row = S->next_cb + S->nnz + 4 * (int64_t)lf;
-    L_ntptr[S->next_vb + S->nv + lf] = (int32_t)row;
-    const int32_t c0 = (int32_t)(S->next_cb + 16 * (int64_t)lf + 2);
-    *reinterpret_cast<int4*>(L_ntids + row) = make_int4(c0, c0 + 4, c0 + 8, c0 + 12);
+  const int32_t v = R.ids[t], w = R.ids[tn], u = R.ids[tp];
+  const int2 c1 = R.ci[t], c0 = R.ci[tp];
+  // every corner's edge id was written by the pass before; the clamp keeps
+  // refined ids inside the refined vertex range whatever it holds
+  const int32_t e1 = clampi(c1.x, emax), e0 = clampi(c0.x, emax);
+  const int32_t s1 = c1.y, s0 = c0.y;
+  int32_t bl1 = 0, bl0 = 0;
+  if (R.blk) {
+    const int32_t bb = R.blk[S->eb];
+    bl1 = R.blk[S->eb + e1] - bb;
+    bl0 = R.blk[S->eb + e0] - bb;
   }
+  const int32_t lt = t - cb, lf = f - fb;
+  const int32_t off = (int32_t)S->id_offset;
+  const int32_t epb = snv + snf + off, fpid = snv + lf + off;
+  reinterpret_cast<int4*>(S->dst_ids)[lt] = make_int4(v - vb + off, epb + e1, fpid, epb + e0);
   if (S->dst_ptr) {  // createQuadColPtr: ptr[t] = 4t
-    int32_t b = 16 * lf;
-    reinterpret_cast<int4*>(S->dst_ptr)[lf] = make_int4(b, b + 4, b + 8, b + 12);
-    if (lf == S->nf - 1) S->dst_ptr[4 * (int64_t)S->nf] = 16 * S->nf;
+    S->dst_ptr[lt] = 4 * lt;
+    if (lt == snnz - 1) S->dst_ptr[snnz] = 4 * snnz;
   }
-  if (S->dst_vals) {  // createQuadVals: vals[i] = i % 4 + 1
-    int4* ov = reinterpret_cast<int4*>(S->dst_vals) + 4 * (int64_t)lf;
-    int4 one = make_int4(1, 2, 3, 4);
-    ov[0] = one;
-    ov[1] = one;
-    ov[2] = one;
-    ov[3] = one;
+  if (S->dst_vals) reinterpret_cast<int4*>(S->dst_vals)[lt] = make_int4(1, 2, 3, 4);  // createQuadVals
+
+  int32_t g1 = f, g0 = f, n1 = 0;
+  if (s1 >= 0) {
+    int32_t x;
+    corner_nbrs<QUAD>(R.fptr, R.cface, s1, g1, n1, x);
+  }
+  if (s0 >= 0) {
+    int32_t x, y;
+    corner_nbrs<QUAD>(R.fptr, R.cface, s0, g0, x, y);
+  }
+  const bool own = s1 < 0 || t < s1;
+  float4* __restrict__ dst = S->dst_verts;
+  if (own) {
+    const bool vlo = v < w;
+    const float4 Pv = R.P[v], Pw = R.P[w];
+    const float4 plo = vlo ? Pv : Pw, phi = vlo ? Pw : Pv;
+    float4 q;
+    if (s1 >= 0) {
+      const float4 fa = dst[snv + ((vlo ? f : g1) - fb)];
+      const float4 fz = dst[snv + ((vlo ? g1 : f) - fb)];
+      q = f4mul(f4add(f4add(f4add(plo, phi), fa), fz), 0.25f);
+    } else {
+      q = f4mul(f4add(plo, phi), 0.5f);
+    }
+    dst[snv + snf + e1] = q;
+  }
+  if (S->next_cb < 0) return;
+
+  // ---- the refined mesh's structure
+  const int32_t ncb = (int32_t)S->next_cb, nvb = (int32_t)S->next_vb;
+  const int32_t c4 = ncb + 4 * lt;
+  const int32_t tw0 = s1 >= 0 ? ncb + 4 * (n1 - cb) + 3 : -1;
+  const int32_t tw1 = ncb + 4 * (tn - cb) + 2;
+  const int32_t tw2 = ncb + 4 * (tp - cb) + 1;
+  const int32_t tw3 = s0 >= 0 ? ncb + 4 * (s0 - cb) : -1;
+  int32_t ex, ey, ez, ew;
+  if (SCHEME == kLAK) {
+    const int32_t base1 = 4 * e1 - bl1, base0 = 4 * e0 - bl0;
+    ex = base1 + (v > w ? 1 : 0);
+    ey = base1 + 2 + ((s1 >= 0 && f > g1) ? 1 : 0);
+    ez = base0 + 2 + ((s0 >= 0 && f > g0) ? 1 : 0);
+    ew = base0 + (v > u ? 1 : 0);
+  } else {
+    ex = 0;  // vertex-owned: written by the vertex pass
+    ey = snnz + (tn - cb);
+    ez = snnz + lt;
+    ew = s0 < 0 ? 2 * snnz + bl0 : 0;  // interior: vertex-owned
+  }
+  int4* __restrict__ o = reinterpret_cast<int4*>(R.nci + c4);
+  o[0] = make_int4(ex, tw0, ey, tw1);
+  o[1] = make_int4(ez, tw2, ew, tw3);
+  // face point row: corner 2 of each refined quad of f, in corner order;
+  // in-neighbour ep(e1)
+  const int32_t nep = nvb + snv + snf;
+  R.ntrow[ncb + snnz + lt] = make_int2(c4 + 2, nep + e1);
+  bool first;
+  if (QUAD) first = (t & 3) == 0;
+  else first = t == R.fptr[f];
+  if (first) R.ntptr[nvb + snv + lf] = ncb + snnz + lt;
+  int32_t* __restrict__ nbnd = R.nbnd;
+  const int32_t neb = (int32_t)S->next_eb;
+  if (SCHEME == kSLAK && nbnd) nbnd[neb + snnz + lt] = 0;  // face point edges are interior
+  if (own) {
+    // edge point row of e1 (the clamp only matters for inconsistent edge
+    // ids: it keeps the row inside the mesh's refined M^T)
+    const int32_t row = min(ncb + 2 * snnz + 4 * e1 - 2 * bl1, ncb + 4 * snnz - (s1 < 0 ? 2 : 4));
+    R.ntptr[nep + e1] = row;
+    int2 a0 = make_int2(c4 + 1, nvb + (v - vb));
+    int2 a1 = make_int2(ncb + 4 * (tn - cb) + 3, nvb + snv + lf);
+    if (a1.x < a0.x) {
+      int2 x = a0;
+      a0 = a1;
+      a1 = x;
+    }
+    int2* __restrict__ q = R.ntrow + row;
+    if (s1 < 0) {
+      q[0] = a0;
+      q[1] = a1;
+    } else {
+      int2 b0 = make_int2(ncb + 4 * (s1 - cb) + 1, nvb + (w - vb));
+      int2 b1 = make_int2(ncb + 4 * (n1 - cb) + 3, nvb + snv + (g1 - fb));
+      if (b1.x < b0.x) {
+        int2 x = b0;
+        b0 = b1;
+        b1 = x;
+      }
+      // merge two sorted pairs
+      const bool alo = a0.x < b0.x, ahi = a1.x > b1.x;
+      const int2 m0 = alo ? a0 : b0, m3 = ahi ? a1 : b1;
+      const int2 l = alo ? b0 : a0, h = ahi ? b1 : a1;
+      const bool lh = l.x < h.x;
+      q[0] = m0;
+      q[1] = lh ? l : h;
+      q[2] = lh ? h : l;
+      q[3] = m3;
+    }
+    if (nbnd) {
+      // the refined mesh has snnz + 2 ne edges
+      const int32_t nlast = neb + snnz + 2 * (emax + 1) - 1;
+      if (SCHEME == kLAK) {
+        const int32_t base1 = min(neb + 4 * e1 - bl1, nlast - (s1 < 0 ? 2 : 3));
+        const int32_t b = s1 < 0 ? 1 : 0;
+        nbnd[base1] = b;
+        nbnd[base1 + 1] = b;
+        nbnd[base1 + 2] = 0;
+        if (s1 >= 0) nbnd[base1 + 3] = 0;
+      } else if (s1 < 0) {
+        nbnd[min(neb + 2 * snnz + bl1, nlast)] = 1;
+      }
+    }
   }
 }
 
-__global__ void k_topology_poly(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr,
-                                const int32_t* __restrict__ eout, int32_t nf, SegTable seg,
-                                int32_t* __restrict__ L_ntptr, int32_t* __restrict__ L_ntids) {
-  int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
-  if (f >= nf) return;
-  const SegDesc* S = seg.d + seg_of(seg.fbase, seg.n, f);
-  const int32_t lf = (int32_t)(f - S->fb);
-  const int32_t vb = (int32_t)S->vb;
-  const int32_t off = (int32_t)S->id_offset;
-  const int32_t eb = S->nv + S->nf + off;
-  const int32_t fpid = S->nv + lf + off;
-  const int64_t cb = S->cb;
-  int32_t s = fptr[f], e = fptr[f + 1];
-  int4* o = reinterpret_cast<int4*>(S->dst_ids);
-  const int32_t emax = S->ne > 0 ? (int32_t)S->ne - 1 : 0;
-  for (int32_t t = s; t < e; ++t) {
-    int32_t tp = (t == s) ? e - 1 : t - 1;
-    int64_t lt = t - cb;
-    int32_t e0 = min(max(eout[t], 0), emax), e1 = min(max(eout[tp], 0), emax);
-    o[lt] = make_int4(ids[t] - vb + off, eb + e0, fpid, eb + e1);
-    if (S->dst_ptr) S->dst_ptr[lt] = (int32_t)(4 * lt);
-    if (S->dst_vals) reinterpret_cast<int4*>(S->dst_vals)[lt] = make_int4(1, 2, 3, 4);
-    if (S->next_cb >= 0) L_ntids[S->next_cb + S->nnz + lt] = (int32_t)(S->next_cb + 4 * lt + 2);
+// vertex: one thread per vertex v with M^T row (t_i, u_i), i < n, sorted.
+//   interior  q = ((sum_i fp(face t_i) + sum_i P[u_i]) / n + P[v] (n - 2)) / n
+//   boundary  q = 3/4 P[v] + 1/8 (neighbours across boundary edges)
+// and for a level that feeds another: the vertex's refined M^T row (corners
+// 4 t_i, in-neighbours ep(e0(t_i))), SLAK ids of the edges it owns and their
+// boundary flags.
+constexpr int kVertexRegs = 4;  // row entries held in registers; longer rows loop
+
+template <bool QUAD, int SCHEME>
+__global__ void __launch_bounds__(kBlock) k_vertex(RefineView R) {
+  const int32_t v = (int32_t)(block_index() * blockDim.x + threadIdx.x);
+  if (v >= R.nv) return;
+  const SegDesc* __restrict__ S = R.seg.d + seg_of(R.seg.vbase, R.seg.n, v);
+  const int32_t vb = (int32_t)S->vb, fb = (int32_t)S->fb, cb = (int32_t)S->cb;
+  const int32_t snv = S->nv;
+  float4* __restrict__ dst = S->dst_verts;
+  const float4* __restrict__ fpts = dst + snv;  // face points, by mesh-local face
+  const float4* __restrict__ P = R.P;
+  const int2* __restrict__ row = R.trow + R.tptr[v];
+  const int n = R.tptr[v + 1] - R.tptr[v];
+  const float4 Pv = P[v];
+  auto face = [&](int32_t t) __attribute__((always_inline)) { return (QUAD ? (t >> 2) : R.cface[t]) - fb; };
+  int2 e[kVertexRegs];
+  float4 a[kVertexRegs], b[kVertexRegs];
+#pragma unroll
+  for (int i = 0; i < kVertexRegs; ++i)
+    if (i < n) e[i] = row[i];
+#pragma unroll
+  for (int i = 0; i < kVertexRegs; ++i)
+    if (i < n) {
+      a[i] = fpts[face(e[i].x)];
+      b[i] = P[e[i].y];
+    }
+  bool boundary = false;
+  if (R.blk)
+    for (int i = 0; i < n; ++i) {
+      int32_t f, tn, tp;
+      corner_nbrs<QUAD>(R.fptr, R.cface, row[i].x, f, tn, tp);
+      boundary |= R.ci[row[i].x].y < 0 || R.ci[tp].y < 0;
+    }
+  float4 q;
+  if (!boundary) {
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < kVertexRegs; ++i)
+      if (i < n) s = f4add(s, a[i]);
+    for (int i = kVertexRegs; i < n; ++i) s = f4add(s, fpts[face(row[i].x)]);
+#pragma unroll
+    for (int i = 0; i < kVertexRegs; ++i)
+      if (i < n) s = f4add(s, b[i]);
+    for (int i = kVertexRegs; i < n; ++i) s = f4add(s, P[row[i].y]);
+    const float nf = (float)n;
+    q = f4div(f4add(f4div(s, nf), f4mul(Pv, nf - 2.0f)), nf);
+  } else {
+    q = f4mul(Pv, 0.75f);
+    for (int i = 0; i < n; ++i) {
+      int32_t f, tn, tp;
+      corner_nbrs<QUAD>(R.fptr, R.cface, row[i].x, f, tn, tp);
+      if (R.ci[row[i].x].y < 0) q = f4add(q, f4mul(P[R.ids[tn]], 0.125f));
+    }
+    for (int i = 0; i < n; ++i) {
+      int32_t f, tn, tp;
+      corner_nbrs<QUAD>(R.fptr, R.cface, row[i].x, f, tn, tp);
+      if (R.ci[tp].y < 0) q = f4add(q, f4mul(P[row[i].y], 0.125f));
+    }
   }
-  if (S->next_cb >= 0) L_ntptr[S->next_vb + S->nv + lf] = (int32_t)(S->next_cb + S->nnz + (s - cb));
-  if (S->dst_ptr && lf == S->nf - 1) S->dst_ptr[S->nnz] = (int32_t)(4 * S->nnz);
+  dst[v - vb] = q;
+  if (S->next_cb < 0) return;
+
+  const int32_t ncb = (int32_t)S->next_cb, nvb = (int32_t)S->next_vb;
+  const int32_t nep = nvb + snv + S->nf;
+  const int32_t emax = S->ne > 0 ? (int32_t)S->ne - 1 : 0;
+  const int32_t pos = R.tptr[v] - cb;  // mesh-local M^T storage position of the row
+  R.ntptr[nvb + (v - vb)] = ncb + pos;
+  int2* __restrict__ nrow = R.ntrow + ncb + pos;
+  int32_t* __restrict__ nci = reinterpret_cast<int32_t*>(R.nci);
+  for (int i = 0; i < n; ++i) {
+    const int32_t t = row[i].x;
+    int32_t f, tn, tp;
+    corner_nbrs<QUAD>(R.fptr, R.cface, t, f, tn, tp);
+    nrow[i] = make_int2(ncb + 4 * (t - cb), nep + clampi(R.ci[tp].x, emax));
+    if (SCHEME == kSLAK) {
+      const int32_t id = pos + i;
+      const int32_t tw = R.ci[t].y;
+      nci[2 * (int64_t)(ncb + 4 * (t - cb))] = id;
+      if (tw >= 0) {
+        int32_t g, twn, twp;
+        corner_nbrs<QUAD>(R.fptr, R.cface, tw, g, twn, twp);
+        nci[2 * (int64_t)(ncb + 4 * (twn - cb) + 3)] = id;
+      }
+      if (R.nbnd) R.nbnd[S->next_eb + id] = tw < 0 ? 1 : 0;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -471,7 +670,7 @@ void init_device_options() {
   static std::once_flag once;
   std::call_once(once, [] {
     const char* e = std::getenv("SLAK_XCD_REMAP");
-    int on = e ? std::atoi(e) : 1;
+    int on = e ? std::atoi(e) : 0;
     (void)hipMemcpyToSymbol(HIP_SYMBOL(g_xcd_remap), &on, sizeof(int));
   });
 }
@@ -569,10 +768,10 @@ void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStre
   SLAK_ROW_DISPATCH(k_symbolic, quad, scheme, cap, g, L, st);
 }
 
-void launch_numeric(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st) {
+void launch_number(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st) {
   if (L.nv <= 0) return;
   unsigned g = grid_for(L.nv);
-  SLAK_ROW_DISPATCH(k_numeric, quad, scheme, cap, g, L, st);
+  SLAK_ROW_DISPATCH(k_number, quad, scheme, cap, g, L, st);
 }
 
 void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st) {
@@ -583,13 +782,28 @@ void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st) {
     k_spgemm_fde<false><<<grid_for(L.nv), kBlock, 0, st>>>(L);
 }
 
-void launch_topology(bool quad, const int32_t* ids, const int32_t* fptr, const int32_t* eout, int32_t nf,
-                     const SegTable& seg, int32_t* ntptr, int32_t* ntids, hipStream_t st) {
-  if (nf <= 0) return;
-  if (quad)
-    k_topology_quad<<<grid_for(nf), kBlock, 0, st>>>(ids, eout, nf, seg, ntptr, ntids);
-  else
-    k_topology_poly<<<grid_for(nf), kBlock, 0, st>>>(ids, fptr, eout, nf, seg, ntptr, ntids);
+void launch_refine(bool quad, int scheme, const RefineView& R, hipStream_t st) {
+  if (R.nnz <= 0) return;
+  unsigned g = grid_for(R.nnz);
+  if (quad) {
+    if (scheme == kLAK) k_refine<true, kLAK><<<g, kBlock, 0, st>>>(R);
+    else k_refine<true, kSLAK><<<g, kBlock, 0, st>>>(R);
+  } else {
+    if (scheme == kLAK) k_refine<false, kLAK><<<g, kBlock, 0, st>>>(R);
+    else k_refine<false, kSLAK><<<g, kBlock, 0, st>>>(R);
+  }
+}
+
+void launch_vertex(bool quad, int scheme, const RefineView& R, hipStream_t st) {
+  if (R.nv <= 0) return;
+  unsigned g = grid_for(R.nv);
+  if (quad) {
+    if (scheme == kLAK) k_vertex<true, kLAK><<<g, kBlock, 0, st>>>(R);
+    else k_vertex<true, kSLAK><<<g, kBlock, 0, st>>>(R);
+  } else {
+    if (scheme == kLAK) k_vertex<false, kLAK><<<g, kBlock, 0, st>>>(R);
+    else k_vertex<false, kSLAK><<<g, kBlock, 0, st>>>(R);
+  }
 }
 
 int64_t scan_partials_len(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }

@@ -194,28 +194,30 @@ void Plan::plan_first_level() {
   L.cnt = arena_.get_n<int32_t>(L.nv);
   L.tptr = arena_.get_n<int32_t>(L.nv + 1);
   L.tids = arena_.get_n<int32_t>(L.nnz);
+  L.trow = arena_.get_n<int2>(L.nnz);
+  L.ci = arena_.get_n<int2>(L.nnz);
   L.ecnt = arena_.get_n<int32_t>(L.nv);
   L.eptr = arena_.get_n<int32_t>(L.nv + 1);
-  L.eout = arena_.get_n<int32_t>(L.nnz);
-  L.bcnt = arena_.get_n<int32_t>(L.nv);
-  L.bptr = arena_.get_n<int32_t>(L.nv + 1);
 }
 
-void Plan::record_analysis(LevelPlan& L, bool count) {
+// First-level analysis: M^T by counting sort, sorted rows with in-neighbours
+// and owned-edge counts (later levels receive their structure from the level
+// before).
+void Plan::record_analysis(LevelPlan& L) {
   const double poly_b = L.quad ? 0.0 : 4.0 * (L.nf + 1) + 4.0 * L.nnz;
-  if (count) {
-    {
-      ProfScope ps(st_, "fill", 4.0 * L.nv);
-      launch_fill_i32(L.cnt, L.nv, 0, st_);
-    }
+  {
+    ProfScope ps(st_, "fill", 4.0 * L.nv);
+    launch_fill_i32(L.cnt, L.nv, 0, st_);
+  }
+  {
     ProfScope ps(st_, L.quad ? "count_quad" : "count_poly", 4.0 * L.nnz + 8.0 * L.nv + (L.quad ? 0.0 : 4.0 * L.nf));
     launch_faces(L.quad, false, true, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), L.cnt, st_);
   }
-  if (count) {  // M^T by counting sort; later levels receive it from the level before
-    {
-      ProfScope ps(st_, "scan", 8.0 * L.nv + 4);
-      launch_scan(L.cnt, L.tptr, L.nv, d_partial_, d_flags_, st_);
-    }
+  {
+    ProfScope ps(st_, "scan", 8.0 * L.nv + 4);
+    launch_scan(L.cnt, L.tptr, L.nv, d_partial_, d_flags_, st_);
+  }
+  {
     ProfScope ps(st_, L.quad ? "transpose_quad" : "transpose_poly",
                  8.0 * L.nnz + 12.0 * L.nv + (L.quad ? 0.0 : 4.0 * L.nf));
     launch_transpose(L.quad, L.ids, L.fptr, (int32_t)L.nf, L.tptr, L.cnt, L.tids, st_);
@@ -230,22 +232,18 @@ void Plan::record_analysis(LevelPlan& L, bool count) {
   V.nnz = L.nnz;
   V.tptr = L.tptr;
   V.tids = L.tids;
+  V.trow = L.trow;
   V.ecnt = L.ecnt;
-  V.bcnt = L.bcnt;
   V.flags = d_flags_;
   V.seg = L.table();
   {
     ProfScope ps(st_, L.quad ? "spgemm_symbolic_quad" : "spgemm_symbolic_poly",
-                 8.0 * L.nnz + 8.0 * L.nv + (L.bcnt ? 4.0 * L.nv : 0.0) + poly_b);
+                 8.0 * L.nnz + 8.0 * L.nv + 8.0 * L.nnz + poly_b);
     launch_symbolic(L.quad, scheme_, L.cap, V, st_);
   }
   {
     ProfScope ps(st_, "scan", 8.0 * L.nv + 4);
     launch_scan(L.ecnt, L.eptr, L.nv, d_partial_, d_flags_, st_);
-  }
-  if (L.bcnt) {
-    ProfScope ps(st_, "scan", 8.0 * L.nv + 4);
-    launch_scan(L.bcnt, L.bptr, L.nv, d_partial_, d_flags_, st_);
   }
 }
 
@@ -306,7 +304,7 @@ void Plan::prepare() {
   if (!all_trusted) fail_if(readback(nullptr, 0, nullptr));
 
   // 2. first-level analysis: M^T, owned-edge counts, per-mesh edge totals
-  record_analysis(L1, true);
+  record_analysis(L1);
   d_bounds_ = arena_.get_n<int32_t>(nseg + 1 + 2 * kRowHistBins);
   launch_seg_bounds(L1.d_vbase, nseg, L1.eptr, d_bounds_, st_);
   launch_row_hist(L1.tptr, (int32_t)L1.nv, L1.fptr, (int32_t)L1.nf, L1.quad ? 4 : 0, d_bounds_ + nseg + 1, st_);
@@ -424,7 +422,7 @@ void Plan::prepare() {
       L.segs.clear();
       L.vbase.clear();
       L.fbase.clear();
-      int64_t vb = 0, fb = 0, cb = 0;
+      int64_t vb = 0, fb = 0, cb = 0, eb = 0;
       for (int j : prev.jobs) {
         MeshJob& J = jobs_[j];
         if (J.levels < l) continue;
@@ -432,6 +430,7 @@ void Plan::prepare() {
         S.vb = vb;
         S.fb = fb;
         S.cb = cb;
+        S.eb = eb;
         S.nv = (int32_t)J.nv[l - 1];
         S.nf = (int32_t)J.nf[l - 1];
         S.nnz = J.nnz[l - 1];
@@ -442,12 +441,14 @@ void Plan::prepare() {
         vb += S.nv;
         fb += S.nf;
         cb += S.nnz;
+        eb += J.ne[l - 1];
       }
       L.vbase.push_back(vb);
       L.fbase.push_back(fb);
       L.nv = vb;
       L.nf = fb;
       L.nnz = cb;
+      L.ne = eb;
       check_i32(L.nv, "union vertices of level " + std::to_string(l));
       check_i32(L.nnz, "union corners of level " + std::to_string(l));
       max_nv = std::max(max_nv, L.nv);
@@ -462,11 +463,13 @@ void Plan::prepare() {
       }
     float4* Un_P = nv_next ? arena_.get_n<float4>(nv_next) : nullptr;
     int32_t* Un_ids = nnz_next ? arena_.get_n<int32_t>(nnz_next) : nullptr;
-    int64_t vbn = 0, cbn = 0;
+    int64_t vbn = 0, cbn = 0, ebn = 0;
     for (size_t k = 0; k < L.segs.size(); ++k) {
       MeshJob& J = jobs_[L.jobs[k]];
       SegDesc& S = L.segs[k];
       S.ne = J.ne[l - 1];
+      if (l == 1) S.eb = bounds[k];
+      L.boundary |= 2 * J.ne[l - 1] != J.nnz[l - 1];  // n_ext = 2 n_e - nnz
       if (J.levels == l) {
         S.dst_verts = J.out_verts;
         S.dst_ids = J.out_ids;
@@ -475,6 +478,7 @@ void Plan::prepare() {
         S.id_offset = 0;
         S.next_vb = -1;
         S.next_cb = -1;
+        S.next_eb = -1;
       } else {
         S.dst_verts = Un_P + vbn;
         S.dst_ids = Un_ids + cbn;
@@ -483,10 +487,13 @@ void Plan::prepare() {
         S.id_offset = vbn;
         S.next_vb = vbn;
         S.next_cb = cbn;
+        S.next_eb = ebn;
         vbn += J.nv[l];
         cbn += J.nnz[l];
+        ebn += J.ne[l];
       }
     }
+    if (l == 1) L.ne = bounds[nseg];
     L.next_nv = nv_next;
     L.next_nnz = nnz_next;
     if (l < Lmax) {
@@ -495,43 +502,40 @@ void Plan::prepare() {
       nx.P = Un_P;
     }
   }
-  // Levels >= 2 receive their M^T from the level before (two alternating
-  // buffers) and share the rest of the scratch, sized for the largest level.
+  // Levels >= 2 receive their structure from the level before (two
+  // alternating buffers of each kind, sized for the largest level); the
+  // boundary flags and their scan are one buffer pair shared by every level.
   if (Lmax >= 2) {
     int32_t* mt_ptr[2] = {arena_.get_n<int32_t>(max_nv + 1), Lmax >= 3 ? arena_.get_n<int32_t>(max_nv + 1) : nullptr};
-    int32_t* mt_ids[2] = {arena_.get_n<int32_t>(max_nnz), Lmax >= 3 ? arena_.get_n<int32_t>(max_nnz) : nullptr};
-    int32_t* ecnt = arena_.get_n<int32_t>(max_nv);
-    int32_t* eptr = arena_.get_n<int32_t>(max_nv + 1);
-    int32_t* eout = arena_.get_n<int32_t>(max_nnz);
-    int32_t* bcnt = nullptr;
-    int32_t* bptr = nullptr;
-    int64_t need2 = scan_partials_len(max_nv);
-    if (need2 > partial_len_) {
-      d_partial_ = arena_.get_n<int64_t>(need2);
-      partial_len_ = need2;
-    }
+    int2* mt_row[2] = {arena_.get_n<int2>(max_nnz), Lmax >= 3 ? arena_.get_n<int2>(max_nnz) : nullptr};
+    int2* mt_ci[2] = {arena_.get_n<int2>(max_nnz), Lmax >= 3 ? arena_.get_n<int2>(max_nnz) : nullptr};
     for (int l = 2; l <= Lmax; ++l) {
       LevelPlan& L = levels_[l - 1];
       L.tptr = mt_ptr[l % 2];
-      L.tids = mt_ids[l % 2];
-      L.ecnt = ecnt;
-      L.eptr = eptr;
-      L.eout = eout;
+      L.trow = mt_row[l % 2];
+      L.ci = mt_ci[l % 2];
       levels_[l - 2].ntptr = L.tptr;
-      levels_[l - 2].ntids = L.tids;
-      bool boundary = false;
-      for (int j : L.jobs) {
-        const MeshJob& J = jobs_[j];
-        boundary |= 2 * J.ne[l - 1] != J.nnz[l - 1];  // n_ext = 2 n_e - nnz
-      }
-      if (boundary) {
-        if (!bcnt) {
-          bcnt = arena_.get_n<int32_t>(max_nv);
-          bptr = arena_.get_n<int32_t>(max_nv + 1);
-        }
-        L.bcnt = bcnt;
-        L.bptr = bptr;
-      }
+      levels_[l - 2].ntrow = L.trow;
+      levels_[l - 2].nci = L.ci;
+    }
+  }
+  int64_t max_ne = 0;
+  for (auto& L : levels_)
+    if (L.boundary) max_ne = std::max(max_ne, L.ne);
+  if (max_ne > 0) {
+    int32_t* bnd = arena_.get_n<int32_t>(max_ne);
+    int32_t* blk = arena_.get_n<int32_t>(max_ne + 1);
+    for (int l = 1; l <= Lmax; ++l) {
+      LevelPlan& L = levels_[l - 1];
+      if (!L.boundary) continue;
+      L.bnd = bnd;
+      L.blk = blk;
+      if (l >= 2) levels_[l - 2].nbnd = bnd;
+    }
+    int64_t need2 = scan_partials_len(max_ne);
+    if (need2 > partial_len_) {
+      d_partial_ = arena_.get_n<int64_t>(need2);
+      partial_len_ = need2;
     }
   }
   for (int l = 1; l <= Lmax; ++l) {
@@ -542,7 +546,7 @@ void Plan::prepare() {
     }
     L.long_rows = l == 1 ? vlong : vlong + flong;
     L.max_row = l == 1 ? vmax : std::max(std::max(vmax, fmax), 4);
-    L.cap = autotune_cap(L.quad, L.nv, L.long_rows, L.max_row);
+    L.cap = autotune_cap(L.quad, L.nv, L.long_rows, L.max_row);  // used by the first level's row kernels
   }
   for (auto& L : levels_) upload_tables(L);
   prepared_ = true;
@@ -551,54 +555,78 @@ void Plan::prepare() {
 void Plan::record_level(int l) {
   LevelPlan& L = levels_[l - 1];
   const double poly_b = L.quad ? 0.0 : 4.0 * (L.nf + 1) + 4.0 * L.nnz;
-  double ne_tot = 0;
-  for (auto& S : L.segs) ne_tot += (double)S.ne;
+  const double ne = (double)L.ne;
+  if (l == 1) {
+    // edge numbering of the first level (later levels get theirs in closed form)
+    LevelView V{};
+    V.ids = L.ids;
+    V.fptr = L.fptr;
+    V.cface = L.cface;
+    V.P = L.P;
+    V.nv = (int32_t)L.nv;
+    V.nf = (int32_t)L.nf;
+    V.nnz = L.nnz;
+    V.tptr = L.tptr;
+    V.tids = L.tids;
+    V.eptr = L.eptr;
+    V.ci = reinterpret_cast<int32_t*>(L.ci);
+    V.bnd = L.bnd;
+    V.flags = d_flags_;
+    V.seg = L.table();
+    ProfScope ps(st_, L.quad ? "spgemm_number_quad" : "spgemm_number_poly",
+                 12.0 * L.nv + 12.0 * L.nnz + 8.0 * L.nnz + (L.bnd ? 4.0 * ne : 0.0) + poly_b);
+    launch_number(L.quad, scheme_, L.cap, V, st_);
+  }
+  if (L.blk) {
+    ProfScope ps(st_, "scan", 8.0 * ne + 4);
+    launch_scan(L.bnd, L.blk, L.ne, d_partial_, d_flags_, st_);
+  }
   {
     ProfScope ps(st_, L.quad ? "facepoints_quad" : "facepoints_poly",
                  4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + (L.quad ? 0.0 : 4.0 * (L.nf + 1)));
     launch_faces(L.quad, true, false, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), nullptr, st_);
   }
-  if (l > 1) record_analysis(L, false);  // M^T came from the level before
-  LevelView V{};
-  V.ids = L.ids;
-  V.fptr = L.fptr;
-  V.cface = L.cface;
-  V.P = L.P;
-  V.nv = (int32_t)L.nv;
-  V.nf = (int32_t)L.nf;
-  V.nnz = L.nnz;
-  V.tptr = L.tptr;
-  V.tids = L.tids;
-  V.eptr = L.eptr;
-  V.eout = L.eout;
-  V.bptr = L.bptr;
-  V.ntptr = L.ntptr;
-  V.ntids = L.ntids;
-  V.flags = d_flags_;
-  V.seg = L.table();
+  RefineView R{};
+  R.ids = L.ids;
+  R.fptr = L.fptr;
+  R.cface = L.cface;
+  R.P = L.P;
+  R.nv = (int32_t)L.nv;
+  R.nf = (int32_t)L.nf;
+  R.nnz = L.nnz;
+  R.tptr = L.tptr;
+  R.trow = L.trow;
+  R.ci = L.ci;
+  R.blk = L.blk;
+  R.seg = L.table();
+  R.ntptr = L.ntptr;
+  R.ntrow = L.ntrow;
+  R.nci = L.nci;
+  R.nbnd = L.nbnd;
   if (L.ntptr) {  // last offset of the next level's M^T
     launch_fill_i32(L.ntptr + L.next_nv, 1, (int32_t)L.next_nnz, st_);
   }
+  const bool next = L.ntptr != nullptr;
   {
-    // reads tptr, tids, ids, eptr, P, face points; writes vertex points, edge
-    // points, eout and the vertex and edge rows of the next level's M^T
-    ProfScope ps(st_, L.quad ? "spgemm_numeric_quad" : "spgemm_numeric_poly",
-                 8.0 * L.nv + 8.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 16.0 * L.nv + 16.0 * ne_tot + 4.0 * L.nnz +
-                     4.0 * L.nv + poly_b + (L.ntptr ? 4.0 * (L.nv + ne_tot) + 12.0 * L.nnz : 0.0));
-    launch_numeric(L.quad, scheme_, L.cap, V, st_);
+    double nnz_final = 0;
+    for (auto& S : L.segs)
+      if (S.dst_ptr) nnz_final += (double)S.nnz;
+    // reads ids, (edge, twin) pairs, positions and face points; writes refined
+    // quads, edge points, the final level's LAK form and the next level's
+    // (edge, twin) pairs, face- and edge-point rows of M^T
+    ProfScope ps(st_, L.quad ? "refine_quad" : "refine_poly",
+                 4.0 * L.nnz + 8.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 16.0 * L.nnz + 16.0 * ne +
+                     20.0 * nnz_final + (L.blk ? 4.0 * ne : 0.0) + poly_b +
+                     (next ? 32.0 * L.nnz + 24.0 * L.nnz + 4.0 * (L.nf + ne) + (L.nbnd ? 16.0 * L.nnz : 0.0) : 0.0));
+    launch_refine(L.quad, scheme_, R, st_);
   }
   {
-    bool lak_final = false;
-    double nnz_final = 0;
-    for (size_t k = 0; k < L.segs.size(); ++k)
-      if (L.segs[k].dst_ptr) {
-        lak_final = true;
-        nnz_final += (double)L.segs[k].nnz;
-      }
-    ProfScope ps(st_, L.quad ? "topology_quad" : "topology_poly",
-                 8.0 * L.nnz + 16.0 * L.nnz + (lak_final ? 20.0 * nnz_final : 0.0) + (L.quad ? 0.0 : 4.0 * L.nf) +
-                     (L.ntptr ? 4.0 * L.nf + 4.0 * L.nnz : 0.0));
-    launch_topology(L.quad, L.ids, L.fptr, L.eout, (int32_t)L.nf, L.table(), L.ntptr, L.ntids, st_);
+    // reads M^T rows, positions and face points; writes vertex points and the
+    // next level's vertex rows of M^T
+    ProfScope ps(st_, L.quad ? "vertexpoints_quad" : "vertexpoints_poly",
+                 4.0 * L.nv + 8.0 * L.nnz + 16.0 * L.nf + 16.0 * L.nv + 16.0 * L.nv + (L.blk ? 8.0 * L.nnz : 0.0) +
+                     poly_b + (next ? 8.0 * L.nnz + 8.0 * L.nnz + 4.0 * L.nv : 0.0));
+    launch_vertex(L.quad, scheme_, R, st_);
   }
 }
 
@@ -617,7 +645,7 @@ void Plan::launch(bool include_analysis) {
       ProfScope ps(st_, "gather_verts", 32.0 * L1.nv);
       launch_gather_verts(L1.table(), (int32_t)L1.nv, P1_, st_);
     }
-    record_analysis(L1, true);
+    record_analysis(L1);
   }
   for (int l = 1; l <= (int)levels_.size(); ++l) record_level(l);
 }

@@ -43,14 +43,24 @@ struct LevelPlan {
   const int32_t* fptr = nullptr;
   const int32_t* cface = nullptr;
   const float4* P = nullptr;
-  int cap = 8;              // M^T row length held in registers (autotuned per level)
-  int64_t long_rows = -1;   // rows longer than 4 (predicted from the first level's histogram)
+  int64_t ne = 0;           // union edges
+  bool boundary = false;    // some mesh of the level has boundary edges
+  int cap = 8;              // first level: M^T row length held in registers (autotuned)
+  int64_t long_rows = -1;   // rows longer than 4 (from the first level's histogram)
   int max_row = -1;
-  // scratch (the first level owns its own; later levels share the largest)
-  int32_t *cnt = nullptr, *tptr = nullptr, *tids = nullptr, *ecnt = nullptr, *eptr = nullptr, *eout = nullptr;
-  int32_t *bcnt = nullptr, *bptr = nullptr;  // owned boundary edges (levels that may have a boundary)
-  // the next level's M^T, written by this level's numeric and topology kernels
-  int32_t *ntptr = nullptr, *ntids = nullptr;
+  // first-level analysis scratch
+  int32_t *cnt = nullptr, *tids = nullptr, *ecnt = nullptr, *eptr = nullptr;
+  // the level's structure: M^T rows with in-neighbours, (edge id, twin) per
+  // corner, boundary flags per edge and their scan (boundary levels)
+  int32_t* tptr = nullptr;
+  int2* trow = nullptr;
+  int2* ci = nullptr;
+  int32_t *bnd = nullptr, *blk = nullptr;
+  // the next level's structure, written by this level's refine and vertex kernels
+  int32_t* ntptr = nullptr;
+  int2* ntrow = nullptr;
+  int2* nci = nullptr;
+  int32_t* nbnd = nullptr;
   int64_t next_nv = 0, next_nnz = 0;  // next level's union sizes
   std::vector<int> jobs;  // segment -> job
   std::vector<SegDesc> segs;
@@ -99,7 +109,7 @@ class Plan {
 
  private:
   void plan_first_level();
-  void record_analysis(LevelPlan& L, bool count);
+  void record_analysis(LevelPlan& L);
   void record_level(int l);
   void upload_tables(LevelPlan& L);
   int readback(const int32_t* src, int count, int64_t* dst);

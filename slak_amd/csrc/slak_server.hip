@@ -11,6 +11,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -61,7 +62,7 @@ uint64_t footprint(const slak_mesh& m, int levels) {
     double nv2 = nv + nf + ne, nnz2 = 4 * nnz;
     double b = 16 * nv + 4 * nnz                  // input
                + 16 * nv2 + 4 * nnz2 * 2 + 4 * nnz  // output ids, LAK ptr/vals
-               + 4 * (4 * nv + 3 * nnz);            // scratch
+               + 16 * nv + 28 * nnz;                // scratch: M^T, rows, (edge, twin), flags
     peak = std::max(peak, b);
     nv = nv2;
     nf = nnz;
@@ -89,12 +90,39 @@ struct slak_server {
   uint64_t steps = 0, completed = 0, faces_done = 0;
   double busy_us = 0;
   int max_batch_seen = 0;
+  // refined corners (and vertices) one level plan of a step may hold: the
+  // int32 index range, or less with SLAK_SERVE_STEP_CORNERS
+  double step_limit = 2147483647.0;
 
   void loop();
   void step(std::vector<std::shared_ptr<Request>>& batch);
+  void step_plan(std::vector<std::shared_ptr<Request>>& batch);
 };
 
+// A level plan indexes its union of meshes with int32: the step runs the
+// batch as consecutive sub-batches whose refined corners and vertices fit.
 void slak_server::step(std::vector<std::shared_ptr<Request>>& batch) {
+  const double lim = step_limit;
+  size_t i = 0;
+  while (i < batch.size()) {
+    size_t j = i;
+    double corners = 0, verts = 0;
+    while (j < batch.size()) {
+      const Request& r = *batch[j];
+      const slak_mesh& m = r.done_levels == 0 ? r.in : r.cur;
+      const double c = 4.0 * (double)m.nnz, v = (double)m.nverts + (double)m.nfaces + (double)m.nnz;
+      if (j > i && (corners + c > lim || verts + v > lim)) break;
+      corners += c;
+      verts += v;
+      ++j;
+    }
+    std::vector<std::shared_ptr<Request>> sub(batch.begin() + i, batch.begin() + j);
+    step_plan(sub);
+    i = j;
+  }
+}
+
+void slak_server::step_plan(std::vector<std::shared_ptr<Request>>& batch) {
   using namespace slak;
   std::vector<MeshJob> jobs;
   jobs.reserve(batch.size());
@@ -228,6 +256,10 @@ int slak_server_start(const slak_server_config* cfg, slak_server** out) {
       throw slak::Error(SLAK_ERR_INVALID_ARGUMENT, "unknown scheme");
     auto s = std::make_unique<slak_server>();
     s->cfg = *cfg;
+    if (const char* e = std::getenv("SLAK_SERVE_STEP_CORNERS")) {
+      const double x = std::atof(e);
+      if (x > 0 && x < s->step_limit) s->step_limit = x;
+    }
     if (s->cfg.max_batch <= 0) s->cfg.max_batch = 1024;
     if (s->cfg.memory_budget_bytes == 0) s->cfg.memory_budget_bytes = (uint64_t)64 << 30;
     SLAK_HIP_CHECK(hipSetDevice(s->cfg.device));

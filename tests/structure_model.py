@@ -1,0 +1,173 @@
+"""Host restatement of the refined-mesh structure rules the HIP refine and
+vertex kernels apply (slak_amd/csrc/slak_kernels.hip, k_refine / k_vertex).
+
+TEST INFRASTRUCTURE: pins the closed forms by which level l writes the
+structure of level l+1 (edge ids of every corner's outgoing half-edge, twin
+corners, sorted M^T rows with in-neighbours, boundary flags per edge) against
+the same structure computed by brute force on the refined mesh, whose faces
+come from the oracle. The brute force (``structure_of``) follows the
+canonical numberings the reference's kernels produce when their atomics are
+serialized: LAK ranks edges by (max, min) vertex (spgemm.cuh:71-100), SLAK by
+the flagged half-edge in (vertex, corner) order (CatmullClarkKernels.cuh:147-236).
+"""
+from __future__ import annotations
+
+from collections import defaultdict
+from dataclasses import dataclass
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+LAK, SLAK = 0, 1
+UNSET = -(1 << 30)
+
+
+@dataclass
+class Structure:
+    ptr: np.ndarray          # nf + 1 face offsets
+    ids: np.ndarray          # nnz corner -> vertex (position order)
+    nv: int
+    eout: np.ndarray         # nnz: edge id of the corner's outgoing half-edge
+    twin: np.ndarray         # nnz: corner of the twin half-edge, -1 on the boundary
+    bnd: np.ndarray          # ne: boundary flag by edge id
+    tptr: np.ndarray         # nv + 1
+    rows: List[Tuple[int, int]]  # nnz (corner, in-neighbour), rows sorted by corner
+
+
+def _corner_tables(ptr: np.ndarray):
+    nf = len(ptr) - 1
+    nnz = int(ptr[-1])
+    cface = np.zeros(nnz, dtype=np.int64)
+    nextc = np.zeros(nnz, dtype=np.int64)
+    prevc = np.zeros(nnz, dtype=np.int64)
+    for f in range(nf):
+        s, e = int(ptr[f]), int(ptr[f + 1])
+        for t in range(s, e):
+            cface[t] = f
+            nextc[t] = s if t + 1 == e else t + 1
+            prevc[t] = e - 1 if t == s else t - 1
+    return cface, nextc, prevc
+
+
+def structure_of(ptr: np.ndarray, ids: np.ndarray, nv: int, scheme: int) -> Structure:
+    """Brute-force structure of a mesh from its faces."""
+    ptr = np.asarray(ptr, dtype=np.int64)
+    ids = np.asarray(ids, dtype=np.int64)
+    cface, nextc, prevc = _corner_tables(ptr)
+    nnz = len(ids)
+    he = {}
+    for t in range(nnz):
+        he[(int(ids[t]), int(ids[nextc[t]]))] = t
+    twin = np.full(nnz, -1, dtype=np.int64)
+    for t in range(nnz):
+        v, w = int(ids[t]), int(ids[nextc[t]])
+        twin[t] = he.get((w, v), -1)
+    rows_by_v = defaultdict(list)
+    for t in range(nnz):
+        rows_by_v[int(ids[t])].append(t)
+    rank = {}
+    if scheme == LAK:
+        edges = sorted({(min(v, w), max(v, w)) for (v, w) in he}, key=lambda e: (e[1], e[0]))
+        rank = {e: i for i, e in enumerate(edges)}
+    else:
+        k = 0
+        for v in range(nv):
+            for t in sorted(rows_by_v[v]):
+                w = int(ids[nextc[t]])
+                if twin[t] < 0 or v < w:
+                    rank[(min(v, w), max(v, w))] = k
+                    k += 1
+    eout = np.array([rank[(min(int(ids[t]), int(ids[nextc[t]])), max(int(ids[t]), int(ids[nextc[t]])))]
+                     for t in range(nnz)], dtype=np.int64)
+    bnd = np.zeros(len(rank), dtype=np.int64)
+    for t in range(nnz):
+        if twin[t] < 0:
+            bnd[eout[t]] = 1
+    tptr = np.zeros(nv + 1, dtype=np.int64)
+    rows = []
+    for v in range(nv):
+        r = sorted(rows_by_v[v])
+        tptr[v + 1] = tptr[v] + len(r)
+        rows.extend((t, int(ids[prevc[t]])) for t in r)
+    return Structure(ptr, ids, nv, eout, twin, bnd, tptr, rows)
+
+
+def refine_structure(S: Structure, scheme: int) -> Structure:
+    """The structure of the refined mesh as the refine (per corner) and
+    vertex (per vertex) kernels write it, in closed form from S."""
+    ptr, ids, nv = S.ptr, S.ids, S.nv
+    nf = len(ptr) - 1
+    nnz = len(ids)
+    ne = len(S.bnd)
+    n_ext = int(S.bnd.sum())
+    cface, nextc, prevc = _corner_tables(ptr)
+    blk = np.concatenate([[0], np.cumsum(S.bnd)]).astype(np.int64)
+    nv2, nnz2, ne2 = nv + nf + ne, 4 * nnz, nnz + 2 * ne
+    nci = np.full((nnz2, 2), UNSET, dtype=np.int64)
+    ntrow: List[Optional[Tuple[int, int]]] = [None] * nnz2
+    ntptr = np.full(nv2 + 1, UNSET, dtype=np.int64)
+    nbnd = np.full(ne2, UNSET, dtype=np.int64)
+    nids = np.zeros(nnz2, dtype=np.int64)
+    # ---- refine: one step per corner t
+    for t in range(nnz):
+        f, tn, tp = int(cface[t]), int(nextc[t]), int(prevc[t])
+        v, w, u = int(ids[t]), int(ids[tn]), int(ids[tp])
+        e1, e0 = int(S.eout[t]), int(S.eout[tp])
+        s1, s0 = int(S.twin[t]), int(S.twin[tp])
+        bl1, bl0 = int(blk[e1]), int(blk[e0])
+        nids[4 * t:4 * t + 4] = [v, nv + nf + e1, nv + f, nv + nf + e0]
+        g1 = int(cface[s1]) if s1 >= 0 else f
+        n1 = int(nextc[s1]) if s1 >= 0 else 0
+        g0 = int(cface[s0]) if s0 >= 0 else f
+        own = s1 < 0 or t < s1
+        tw = [4 * n1 + 3 if s1 >= 0 else -1, 4 * tn + 2, 4 * tp + 1, 4 * s0 if s0 >= 0 else -1]
+        if scheme == LAK:
+            b1, b0 = 4 * e1 - bl1, 4 * e0 - bl0
+            eo = [b1 + (v > w), b1 + 2 + (s1 >= 0 and f > g1), b0 + 2 + (s0 >= 0 and f > g0), b0 + (v > u)]
+        else:
+            eo = [UNSET, nnz + tn, nnz + t, 2 * nnz + bl0 if s0 < 0 else UNSET]
+        for k in range(4):
+            nci[4 * t + k] = (eo[k], tw[k])
+        ntrow[nnz + t] = (4 * t + 2, nv + nf + e1)
+        if t == int(ptr[f]):
+            ntptr[nv + f] = nnz + t
+        if scheme == SLAK:
+            nbnd[nnz + t] = 0
+        if own:
+            row = 2 * nnz + 4 * e1 - 2 * bl1
+            ntptr[nv + nf + e1] = row
+            ent = [(4 * t + 1, v), (4 * tn + 3, nv + f)]
+            if s1 >= 0:
+                ent += [(4 * s1 + 1, w), (4 * n1 + 3, nv + g1)]
+            for i, x in enumerate(sorted(ent)):
+                ntrow[row + i] = x
+            if scheme == LAK:
+                flags = [1, 1, 0] if s1 < 0 else [0, 0, 0, 0]
+                for k, fl in enumerate(flags):
+                    nbnd[b1_of(e1, bl1) + k] = fl
+            elif s1 < 0:
+                nbnd[2 * nnz + bl1] = 1
+    # ---- vertex: one step per vertex
+    for v in range(nv):
+        pos = int(S.tptr[v])
+        ntptr[v] = pos
+        for i in range(int(S.tptr[v + 1]) - pos):
+            t, _ = S.rows[pos + i]
+            ntrow[pos + i] = (4 * t, nv + nf + int(S.eout[prevc[t]]))
+            if scheme == SLAK:
+                nci[4 * t, 0] = pos + i
+                tw = int(S.twin[t])
+                if tw >= 0:
+                    nci[4 * int(nextc[tw]) + 3, 0] = pos + i
+                nbnd[pos + i] = 1 if tw < 0 else 0
+    ntptr[nv2] = nnz2
+    assert (nci != UNSET).all() and all(r is not None for r in ntrow)
+    assert (ntptr != UNSET).all() and (nbnd != UNSET).all()
+    assert int(nbnd.sum()) == 2 * n_ext
+    return Structure(np.arange(nnz + 1, dtype=np.int64) * 4, nids, nv2, nci[:, 0], nci[:, 1], nbnd, ntptr,
+                     [tuple(map(int, r)) for r in ntrow])
+
+
+def b1_of(e: int, bl: int) -> int:
+    """First edge id of the refined mesh owned by the edge point of e (LAK)."""
+    return 4 * e - bl

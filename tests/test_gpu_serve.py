@@ -23,8 +23,14 @@ def fetch(m):
     return ids, verts
 
 
+@pytest.mark.parametrize("step_corners", [None, 300000], ids=["one-plan-per-step", "split-steps"])
 @pytest.mark.parametrize("scheme", [LAK, SLAK], ids=["LAK", "SLAK"])
-def test_alone_equals_in_mix(scheme):
+def test_alone_equals_in_mix(scheme, step_corners, monkeypatch):
+    """A step whose union would exceed the per-plan index range runs as several
+    plans (forced here with a small SLAK_SERVE_STEP_CORNERS); results are the
+    same either way."""
+    if step_corners:
+        monkeypatch.setenv("SLAK_SERVE_STEP_CORNERS", str(step_corners))
     rng = np.random.default_rng(5)
     hosts, levels = [], []
     for i in range(30):
