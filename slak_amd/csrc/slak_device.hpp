@@ -38,6 +38,10 @@ enum : int {
   kFlagDegenerate = 64,
   kFlagOverflow = 128,
 };
+// flags the first-level gather raises for an invalid input mesh; the analysis
+// kernels after it do nothing once one is set (their indices would come from
+// the invalid mesh)
+constexpr int kFlagsInput = kFlagBadPtr | kFlagBadOrder | kFlagBadId | kFlagBadVals | kFlagBadQuad;
 
 // One mesh of a level's union.
 struct SegDesc {
@@ -69,6 +73,7 @@ struct SegTable {
   const int64_t* fbase;  // n + 1 face bases
   int32_t n;
   int* mflags;           // per-segment validation flags
+  const int64_t* cbase;  // n + 1 corner bases
 };
 
 // Segment of a union index: the last k with base[k] <= x (empty segments skipped).
@@ -96,6 +101,7 @@ struct LevelView {
   const int32_t* tptr;
   int32_t* tids;         // M^T corners (first level: transposed, rows sorted on use)
   int2* trow;            // sorted rows with in-neighbours (written by the symbolic pass), or null
+  int32_t* mpos;         // M^T storage position of every corner (SLAK, symbolic pass), or null
   int32_t* ecnt;
   const int32_t* eptr;
   int32_t* ci;           // (edge id, twin) pairs per corner, written by the number pass
@@ -426,12 +432,16 @@ struct RefineView {
   const int2* trow;      // (corner, in-neighbour), sorted by corner
   const int2* ci;        // (mesh-local edge id of the outgoing half-edge, twin corner or -1)
   const int32_t* blk;    // exclusive scan of the union edge boundary flags, or null: closed level
+  const int32_t* mpos;   // M^T storage position of every corner (SLAK)
   SegTable seg;
   // next level (null when no mesh continues)
   int32_t* ntptr;
   int2* ntrow;
   int2* nci;
   int32_t* nbnd;         // next level's union edge boundary flags, or null: closed
+  int32_t* nmpos;        // next level's M^T positions (SLAK)
+  int32_t next_nv;       // next level's union vertices and corners (end of its M^T)
+  int32_t next_nnz;
 };
 
 // Face, successor and predecessor corner of corner t (quad: bit arithmetic;

@@ -38,23 +38,29 @@ struct Error : std::runtime_error {
 // reads SLAK_XCD_REMAP once and sets the device-side option (call before capture)
 void init_device_options();
 void launch_fill_i32(int32_t* p, int64_t n, int32_t value, hipStream_t st);
+// also zeroes zero[0..nzero) (the counters of the analysis that follows)
 void launch_gather(const SegTable& seg, int32_t nf_total, bool use_vals, int32_t* ids_pos, int32_t* fptr,
-                   int32_t* cface, int* flags, hipStream_t st);
+                   int32_t* cface, int* flags, int32_t* zero, int32_t nzero, hipStream_t st);
 void launch_gather_verts(const SegTable& seg, int32_t nv_total, float4* P, hipStream_t st);
 constexpr int kRowHistBins = 17;  // row lengths 0..15, 16 = longer
+// hist must be zeroed; also writes bounds[k] = eptr[vbase[k]], k = 0..nseg
 void launch_row_hist(const int32_t* tptr, int32_t nv, const int32_t* fptr, int32_t nf, int32_t face_order,
-                     int32_t* hist, hipStream_t st);
+                     int32_t* hist, const int64_t* vbase, int32_t nseg, const int32_t* eptr, int32_t* bounds,
+                     hipStream_t st);
 void launch_copy_probe(const float4* a, float4* b, int64_t n, hipStream_t st);
-void launch_seg_bounds(const int64_t* vbase, int32_t n, const int32_t* eptr, int32_t* out, hipStream_t st);
+// guard (or null): the kernel does nothing when it holds an input-validation flag
 void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int32_t* fptr, const float4* P, int32_t nf,
-                  const SegTable& seg, int32_t* cnt, hipStream_t st);
+                  const SegTable& seg, int32_t* cnt, const int* guard, hipStream_t st);
 void launch_transpose(bool quad, const int32_t* ids, const int32_t* fptr, int32_t nf, const int32_t* tptr,
-                      int32_t* cnt, int32_t* tids, hipStream_t st);
+                      int32_t* cnt, int32_t* tids, const int* guard, hipStream_t st);
 void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
 void launch_number(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
 void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st);
 void launch_refine(bool quad, int scheme, const RefineView& R, hipStream_t st);
 void launch_vertex(bool quad, int scheme, const RefineView& R, hipStream_t st);
+// LAK form (ptr, vals) of every segment of a level with dst_ptr set; nnz =
+// the level's union corners (= refined faces)
+void launch_lak_form(const SegTable& seg, int64_t nnz, hipStream_t st);
 int64_t scan_partials_len(int64_t n);
 void launch_scan(const int32_t* in, int32_t* out, int64_t n, int64_t* partial, int* flags, hipStream_t st);
 
@@ -82,6 +88,15 @@ class Pool {
 };
 
 Pool& pool();
+
+// A second stream of the calling thread on the current device (default
+// priority, non-blocking) with a fork and a join event, for work that runs
+// beside a plan's chain. Lives until the thread ends.
+struct AuxStream {
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+AuxStream& aux_stream();
 
 // Allocations of one library call: released back to the pool after the
 // call's final stream synchronization, except buffers handed to the caller.

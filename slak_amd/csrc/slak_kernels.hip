@@ -83,8 +83,10 @@ __global__ void __launch_bounds__(kBlock) k_copy_probe(const float4* __restrict_
 // position order (sortKeyValuePairsSegmentedInPlace, LAKCatmullClarkKernels.cuh:147-180)
 
 __global__ void k_gather(SegTable seg, int32_t nf_total, int use_vals, int32_t* __restrict__ ids_pos,
-                         int32_t* __restrict__ fptr, int32_t* __restrict__ cface, int* __restrict__ flags) {
+                         int32_t* __restrict__ fptr, int32_t* __restrict__ cface, int* __restrict__ flags,
+                         int32_t* __restrict__ zero, int32_t nzero) {
   int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
+  if (f < nzero) zero[f] = 0;  // counters of the analysis that follows
   if (f >= nf_total) return;
   const int k = seg_of(seg.fbase, seg.n, f);
   const SegDesc* S = seg.d + k;
@@ -151,14 +153,20 @@ __global__ void k_gather_verts(SegTable seg, int32_t nv_total, float4* __restric
 }
 
 // Row-length histograms of a level: M^T rows (valence, bins 0..15, 16 = more)
-// and face orders (same bins), block-local in LDS then one atomic per bin.
+// and face orders (same bins), block-local in LDS then one atomic per bin
+// into a zeroed hist. Block 0 also writes eptr at every segment's first
+// vertex: bounds[k] = eptr[vbase[k]], k = 0..n.
 constexpr int kHistBins = 17;
 
 __global__ void __launch_bounds__(kBlock) k_row_hist(const int32_t* __restrict__ tptr, int32_t nv,
                                                       const int32_t* __restrict__ fptr, int32_t nf, int32_t face_order,
-                                                      int32_t* __restrict__ hist) {
+                                                      int32_t* __restrict__ hist, const int64_t* __restrict__ vbase,
+                                                      int32_t nseg, const int32_t* __restrict__ eptr,
+                                                      int32_t* __restrict__ bounds) {
   __shared__ int32_t h[2 * kHistBins];
   for (int i = threadIdx.x; i < 2 * kHistBins; i += blockDim.x) h[i] = 0;
+  if (blockIdx.x == 0)
+    for (int k = threadIdx.x; k <= nseg; k += blockDim.x) bounds[k] = eptr[vbase[k]];
   __syncthreads();
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -175,22 +183,15 @@ __global__ void __launch_bounds__(kBlock) k_row_hist(const int32_t* __restrict__
     if (h[b]) atomicAdd(&hist[b], h[b]);
 }
 
-// eptr at every segment's first vertex: out[k] = eptr[vbase[k]], k = 0..n
-__global__ void k_seg_bounds(const int64_t* __restrict__ vbase, int32_t n, const int32_t* __restrict__ eptr,
-                             int32_t* __restrict__ out) {
-  int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k > n) return;
-  out[k] = eptr[vbase[k]];
-}
-
 // ---------------------------------------------------------------------------
 // valence count (+ face points into each segment's destination)
 
 template <bool QUAD, bool FP, bool COUNT>
 __global__ void k_faces(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr, const float4* __restrict__ P,
-                        int32_t nf, SegTable seg, int32_t* __restrict__ cnt) {
+                        int32_t nf, SegTable seg, int32_t* __restrict__ cnt, const int* __restrict__ guard) {
   int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (f >= nf) return;
+  if (guard && (*guard & kFlagsInput)) return;
   float4* dst = nullptr;
   if (FP) {
     const SegDesc* S = seg.d + seg_of(seg.fbase, seg.n, f);
@@ -222,9 +223,11 @@ __global__ void k_faces(const int32_t* __restrict__ ids, const int32_t* __restri
 
 template <bool QUAD>
 __global__ void k_transpose(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr, int32_t nf,
-                            const int32_t* __restrict__ tptr, int32_t* __restrict__ cnt, int32_t* __restrict__ tids) {
+                            const int32_t* __restrict__ tptr, int32_t* __restrict__ cnt, int32_t* __restrict__ tids,
+                            const int* __restrict__ guard) {
   int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (f >= nf) return;
+  if (guard && (*guard & kFlagsInput)) return;
   if (QUAD) {
     int4 c = reinterpret_cast<const int4*>(ids)[f];
     int32_t t = 4 * f;
@@ -259,6 +262,7 @@ template <bool QUAD, int SCHEME, int CAP>
 __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
   int32_t v = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (v >= L.nv) return;
+  if (*L.flags & kFlagsInput) return;
   int32_t s = L.tptr[v];
   int n = L.tptr[v + 1] - s;
   int32_t count = 0, bcount = 0;
@@ -272,12 +276,18 @@ __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
 #pragma unroll
       for (int i = 0; i < CAP; ++i)
         if (i < n) trow[s + i] = make_int2(r.T(i), r.U(i));
+    if (L.mpos)
+#pragma unroll
+      for (int i = 0; i < CAP; ++i)
+        if (i < n) L.mpos[r.T(i)] = s + i;
   } else {
     sort_row_in_place(L.tids + s, n);
     MemRow<QUAD> r(L, s, n);
     row_symbolic<SCHEME>(r, v, count, bcount, bad);
     if (trow)
       for (int i = 0; i < n; ++i) trow[s + i] = make_int2(r.T(i), r.U(i));
+    if (L.mpos)
+      for (int i = 0; i < n; ++i) L.mpos[r.T(i)] = s + i;
   }
   L.ecnt[v] = count;
   (void)bcount;
@@ -322,108 +332,148 @@ __global__ void __launch_bounds__(kBlock) k_spgemm_fde(LevelView L) {
   }
 }
 
-// ---------------------------------------------------------------------------
-// refine: one thread per corner t of face f, v -> w the corner's out-half-edge
-// (edge e1, twin s1), u -> v the in-half-edge (edge e0, twin s0).
-//   refined quad t = (v, ep(e1), fp(f), ep(e0))   in the mesh's local numbering
-//   edge point e1  (written by the half-edge min(t, s1), or t on the boundary)
-//                  = (P_lo + P_hi + fp(face of lo->hi) + fp(face of hi->lo)) / 4,
-//                    (P_lo + P_hi) / 2 on the boundary
-// and for a level that feeds another, in closed form:
-//   twins          4t -> 4 next(s1) + 3, 4t+1 -> 4 next(t) + 2,
-//                  4t+2 -> 4 prev(t) + 1, 4t+3 -> 4 s0
-//   LAK edge ids   edge point e owns its edges to the two end points, then to
-//                  the face points, ranked by the smaller id: base(e) = 4e - bl(e)
-//   SLAK edge ids  vertices own one edge per corner (M^T storage order), face
-//                  points one per corner (nnz + corner), boundary edge points
-//                  one (2 nnz + bl(e)); the vertex-owned ones are filled in by
-//                  the vertex pass, which knows the M^T rank
-//   M^T rows       face point f: its corners 4t+2; edge point e: 4t+1,
-//                  4 next(t) + 3 of both half-edges, sorted
-// bl(e) = boundary edges with a smaller id (0 on closed levels).
-
 __device__ __forceinline__ int32_t clampi(int32_t x, int32_t hi) { return min(max(x, 0), hi); }
 
-template <bool QUAD, int SCHEME>
-__global__ void __launch_bounds__(kBlock) k_refine(RefineView R) {
-  const int64_t t64 = block_index() * blockDim.x + threadIdx.x;
-  if (t64 >= R.nnz) return;
-  const int32_t t = (int32_t)t64;
-  int32_t f, tn, tp;
-  corner_nbrs<QUAD>(R.fptr, R.cface, t, f, tn, tp);
-  const SegDesc* __restrict__ S = R.seg.d + seg_of(R.seg.fbase, R.seg.n, f);
-  const int32_t vb = (int32_t)S->vb, fb = (int32_t)S->fb, cb = (int32_t)S->cb;
-  const int32_t snv = S->nv, snf = S->nf;
-  const int32_t snnz = (int32_t)S->nnz;
-  const int32_t emax = S->ne > 0 ? (int32_t)S->ne - 1 : 0;
-  const int32_t v = R.ids[t], w = R.ids[tn], u = R.ids[tp];
-  const int2 c1 = R.ci[t], c0 = R.ci[tp];
-  // every corner's edge id was written by the pass before; the clamp keeps
-  // refined ids inside the refined vertex range whatever it holds
-  const int32_t e1 = clampi(c1.x, emax), e0 = clampi(c0.x, emax);
-  const int32_t s1 = c1.y, s0 = c0.y;
-  int32_t bl1 = 0, bl0 = 0;
-  if (R.blk) {
-    const int32_t bb = R.blk[S->eb];
-    bl1 = R.blk[S->eb + e1] - bb;
-    bl0 = R.blk[S->eb + e0] - bb;
-  }
-  const int32_t lt = t - cb, lf = f - fb;
-  const int32_t off = (int32_t)S->id_offset;
-  const int32_t epb = snv + snf + off, fpid = snv + lf + off;
-  reinterpret_cast<int4*>(S->dst_ids)[lt] = make_int4(v - vb + off, epb + e1, fpid, epb + e0);
-  if (S->dst_ptr) {  // createQuadColPtr: ptr[t] = 4t
-    S->dst_ptr[lt] = 4 * lt;
-    if (lt == snnz - 1) S->dst_ptr[snnz] = 4 * snnz;
-  }
-  if (S->dst_vals) reinterpret_cast<int4*>(S->dst_vals)[lt] = make_int4(1, 2, 3, 4);  // createQuadVals
+// Streamed outputs of the final level are written once and not read back by
+// this pipeline: non-temporal stores keep them from displacing the gathered
+// positions in the caches.
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_nt(int32_t* p, int a, int b, int c, int d) {
+  v4i_t x = {a, b, c, d};
+  __builtin_nontemporal_store(x, reinterpret_cast<v4i_t*>(p));
+}
+__device__ __forceinline__ void store_nt(int32_t* p, int a) { __builtin_nontemporal_store(a, p); }
 
-  int32_t g1 = f, g0 = f, n1 = 0;
-  if (s1 >= 0) {
+// Per-segment constants of a refine thread.
+struct SegK {
+  const SegDesc* S;
+  int32_t vb, fb, cb, snv, snf, snnz, emax, off;
+  bool next;
+};
+
+__device__ __forceinline__ SegK seg_consts(const SegTable& seg, int32_t f) {
+  SegK K;
+  K.S = seg.d + seg_of(seg.fbase, seg.n, f);
+  K.vb = (int32_t)K.S->vb;
+  K.fb = (int32_t)K.S->fb;
+  K.cb = (int32_t)K.S->cb;
+  K.snv = K.S->nv;
+  K.snf = K.S->nf;
+  K.snnz = (int32_t)K.S->nnz;
+  K.emax = K.S->ne > 0 ? (int32_t)K.S->ne - 1 : 0;
+  K.off = (int32_t)K.S->id_offset;
+  K.next = K.S->next_cb >= 0;
+  return K;
+}
+
+// Everything one corner's stores need, loaded beforehand.
+struct CornerIn {
+  int32_t t, f, tn, tp;   // corner, face, successor / predecessor corner
+  int32_t v, w, u;        // its vertex, out- and in-neighbour
+  int32_t e1, e0, s1, s0; // out / in edge ids and twin corners
+  int32_t g1, g0, n1, ns0;  // twin faces, successors of the twins
+  int32_t bl1, bl0;       // boundary edges before e1 / e0
+  int32_t mp_t, mp_s0;    // M^T positions of t and s0 (SLAK, next level)
+  bool own;               // this half-edge writes edge e1
+  float4 plo, phi, fa, fz;  // edge point inputs (own only)
+};
+
+// First round of a corner's loads: ids and (edge, twin) pairs already in
+// hand; derives twins' faces and decides ownership.
+template <bool QUAD>
+__device__ __forceinline__ void corner_topology(const RefineView& R, const SegK& K, CornerIn& c, int2 c1, int2 c0) {
+  c.e1 = clampi(c1.x, K.emax);
+  c.e0 = clampi(c0.x, K.emax);
+  c.s1 = c1.y;
+  c.s0 = c0.y;
+  c.g1 = c.f;
+  c.g0 = c.f;
+  c.n1 = 0;
+  c.ns0 = 0;
+  if (c.s1 >= 0) {
     int32_t x;
-    corner_nbrs<QUAD>(R.fptr, R.cface, s1, g1, n1, x);
+    corner_nbrs<QUAD>(R.fptr, R.cface, c.s1, c.g1, c.n1, x);
   }
-  if (s0 >= 0) {
-    int32_t x, y;
-    corner_nbrs<QUAD>(R.fptr, R.cface, s0, g0, x, y);
+  if (c.s0 >= 0) {
+    int32_t x;
+    corner_nbrs<QUAD>(R.fptr, R.cface, c.s0, c.g0, c.ns0, x);
   }
-  const bool own = s1 < 0 || t < s1;
-  float4* __restrict__ dst = S->dst_verts;
-  if (own) {
-    const bool vlo = v < w;
-    const float4 Pv = R.P[v], Pw = R.P[w];
-    const float4 plo = vlo ? Pv : Pw, phi = vlo ? Pw : Pv;
+  c.own = c.s1 < 0 || c.t < c.s1;
+}
+
+// Second round: boundary ranks and SLAK positions.
+template <int SCHEME>
+__device__ __forceinline__ void corner_ranks(const RefineView& R, const SegK& K, CornerIn& c) {
+  c.bl1 = 0;
+  c.bl0 = 0;
+  if (R.blk) {
+    const int32_t bb = R.blk[K.S->eb];
+    c.bl1 = R.blk[K.S->eb + c.e1] - bb;
+    c.bl0 = R.blk[K.S->eb + c.e0] - bb;
+  }
+  c.mp_t = 0;
+  c.mp_s0 = 0;
+  if (SCHEME == kSLAK && K.next) {
+    c.mp_t = R.mpos[c.t];
+    if (c.s0 >= 0) c.mp_s0 = R.mpos[c.s0];
+  }
+}
+
+// Every store of one corner t: refined quad, LAK form, edge point (owner)
+// and, for a level that feeds another, the refined mesh's structure.
+template <bool QUAD, int SCHEME>
+__device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K, const CornerIn& c) {
+  const SegDesc* __restrict__ S = K.S;
+  const int32_t lt = c.t - K.cb, lf = c.f - K.fb;
+  const int32_t epb = K.snv + K.snf + K.off, fpid = K.snv + lf + K.off;
+  const bool final_level = !K.next;
+  if (final_level) {
+    store_nt(S->dst_ids + 4 * lt, c.v - K.vb + K.off, epb + c.e1, fpid, epb + c.e0);
+  } else {
+    reinterpret_cast<int4*>(S->dst_ids)[lt] = make_int4(c.v - K.vb + K.off, epb + c.e1, fpid, epb + c.e0);
+  }
+  if (c.own) {
     float4 q;
-    if (s1 >= 0) {
-      const float4 fa = dst[snv + ((vlo ? f : g1) - fb)];
-      const float4 fz = dst[snv + ((vlo ? g1 : f) - fb)];
-      q = f4mul(f4add(f4add(f4add(plo, phi), fa), fz), 0.25f);
-    } else {
-      q = f4mul(f4add(plo, phi), 0.5f);
-    }
-    dst[snv + snf + e1] = q;
+    if (c.s1 >= 0)
+      q = f4mul(f4add(f4add(f4add(c.plo, c.phi), c.fa), c.fz), 0.25f);
+    else
+      q = f4mul(f4add(c.plo, c.phi), 0.5f);
+    S->dst_verts[K.snv + K.snf + c.e1] = q;
   }
-  if (S->next_cb < 0) return;
+  if (!K.next) return;
 
   // ---- the refined mesh's structure
+  const int32_t cb = K.cb, snnz = K.snnz, snv = K.snv, snf = K.snf;
   const int32_t ncb = (int32_t)S->next_cb, nvb = (int32_t)S->next_vb;
   const int32_t c4 = ncb + 4 * lt;
-  const int32_t tw0 = s1 >= 0 ? ncb + 4 * (n1 - cb) + 3 : -1;
-  const int32_t tw1 = ncb + 4 * (tn - cb) + 2;
-  const int32_t tw2 = ncb + 4 * (tp - cb) + 1;
-  const int32_t tw3 = s0 >= 0 ? ncb + 4 * (s0 - cb) : -1;
+  const int32_t tw0 = c.s1 >= 0 ? ncb + 4 * (c.n1 - cb) + 3 : -1;
+  const int32_t tw1 = ncb + 4 * (c.tn - cb) + 2;
+  const int32_t tw2 = ncb + 4 * (c.tp - cb) + 1;
+  const int32_t tw3 = c.s0 >= 0 ? ncb + 4 * (c.s0 - cb) : -1;
+  // edge point rows (the clamp only matters for inconsistent edge ids: it
+  // keeps a row inside the mesh's refined M^T)
+  const int32_t row1 = min(ncb + 2 * snnz + 4 * c.e1 - 2 * c.bl1, ncb + 4 * snnz - (c.s1 < 0 ? 2 : 4));
+  const int32_t row0 = min(ncb + 2 * snnz + 4 * c.e0 - 2 * c.bl0, ncb + 4 * snnz - (c.s0 < 0 ? 2 : 4));
   int32_t ex, ey, ez, ew;
   if (SCHEME == kLAK) {
-    const int32_t base1 = 4 * e1 - bl1, base0 = 4 * e0 - bl0;
-    ex = base1 + (v > w ? 1 : 0);
-    ey = base1 + 2 + ((s1 >= 0 && f > g1) ? 1 : 0);
-    ez = base0 + 2 + ((s0 >= 0 && f > g0) ? 1 : 0);
-    ew = base0 + (v > u ? 1 : 0);
+    const int32_t base1 = 4 * c.e1 - c.bl1, base0 = 4 * c.e0 - c.bl0;
+    ex = base1 + (c.v > c.w ? 1 : 0);
+    ey = base1 + 2 + ((c.s1 >= 0 && c.f > c.g1) ? 1 : 0);
+    ez = base0 + 2 + ((c.s0 >= 0 && c.f > c.g0) ? 1 : 0);
+    ew = base0 + (c.v > c.u ? 1 : 0);
   } else {
-    ex = 0;  // vertex-owned: written by the vertex pass
-    ey = snnz + (tn - cb);
+    ex = c.mp_t - cb;
+    ey = snnz + (c.tn - cb);
     ez = snnz + lt;
-    ew = s0 < 0 ? 2 * snnz + bl0 : 0;  // interior: vertex-owned
+    ew = c.s0 < 0 ? 2 * snnz + c.bl0 : c.mp_s0 - cb;
+    // M^T positions of the refined corners
+    const int32_t a1 = c4 + 1, a3 = c4 + 3;
+    int32_t r1 = (ncb + 4 * (c.tn - cb) + 3 < a1) ? 1 : 0;
+    if (c.s1 >= 0) r1 += (ncb + 4 * (c.s1 - cb) + 1 < a1 ? 1 : 0) + (ncb + 4 * (c.n1 - cb) + 3 < a1 ? 1 : 0);
+    int32_t r3 = (ncb + 4 * (c.tp - cb) + 1 < a3) ? 1 : 0;
+    if (c.s0 >= 0) r3 += (ncb + 4 * (c.s0 - cb) + 1 < a3 ? 1 : 0) + (ncb + 4 * (c.ns0 - cb) + 3 < a3 ? 1 : 0);
+    reinterpret_cast<int4*>(R.nmpos)[c4 >> 2] =
+        make_int4(ncb + (c.mp_t - cb), row1 + r1, ncb + snnz + lt, row0 + r3);
   }
   int4* __restrict__ o = reinterpret_cast<int4*>(R.nci + c4);
   o[0] = make_int4(ex, tw0, ey, tw1);
@@ -431,33 +481,30 @@ __global__ void __launch_bounds__(kBlock) k_refine(RefineView R) {
   // face point row: corner 2 of each refined quad of f, in corner order;
   // in-neighbour ep(e1)
   const int32_t nep = nvb + snv + snf;
-  R.ntrow[ncb + snnz + lt] = make_int2(c4 + 2, nep + e1);
+  R.ntrow[ncb + snnz + lt] = make_int2(c4 + 2, nep + c.e1);
   bool first;
-  if (QUAD) first = (t & 3) == 0;
-  else first = t == R.fptr[f];
+  if (QUAD) first = (c.t & 3) == 0;
+  else first = c.t == R.fptr[c.f];
   if (first) R.ntptr[nvb + snv + lf] = ncb + snnz + lt;
   int32_t* __restrict__ nbnd = R.nbnd;
   const int32_t neb = (int32_t)S->next_eb;
   if (SCHEME == kSLAK && nbnd) nbnd[neb + snnz + lt] = 0;  // face point edges are interior
-  if (own) {
-    // edge point row of e1 (the clamp only matters for inconsistent edge
-    // ids: it keeps the row inside the mesh's refined M^T)
-    const int32_t row = min(ncb + 2 * snnz + 4 * e1 - 2 * bl1, ncb + 4 * snnz - (s1 < 0 ? 2 : 4));
-    R.ntptr[nep + e1] = row;
-    int2 a0 = make_int2(c4 + 1, nvb + (v - vb));
-    int2 a1 = make_int2(ncb + 4 * (tn - cb) + 3, nvb + snv + lf);
+  if (c.own) {
+    R.ntptr[nep + c.e1] = row1;
+    int2 a0 = make_int2(c4 + 1, nvb + (c.v - K.vb));
+    int2 a1 = make_int2(ncb + 4 * (c.tn - cb) + 3, nvb + snv + lf);
     if (a1.x < a0.x) {
       int2 x = a0;
       a0 = a1;
       a1 = x;
     }
-    int2* __restrict__ q = R.ntrow + row;
-    if (s1 < 0) {
+    int2* __restrict__ q = R.ntrow + row1;
+    if (c.s1 < 0) {
       q[0] = a0;
       q[1] = a1;
     } else {
-      int2 b0 = make_int2(ncb + 4 * (s1 - cb) + 1, nvb + (w - vb));
-      int2 b1 = make_int2(ncb + 4 * (n1 - cb) + 3, nvb + snv + (g1 - fb));
+      int2 b0 = make_int2(ncb + 4 * (c.s1 - cb) + 1, nvb + (c.w - K.vb));
+      int2 b1 = make_int2(ncb + 4 * (c.n1 - cb) + 3, nvb + snv + (c.g1 - K.fb));
       if (b1.x < b0.x) {
         int2 x = b0;
         b0 = b1;
@@ -475,30 +522,82 @@ __global__ void __launch_bounds__(kBlock) k_refine(RefineView R) {
     }
     if (nbnd) {
       // the refined mesh has snnz + 2 ne edges
-      const int32_t nlast = neb + snnz + 2 * (emax + 1) - 1;
+      const int32_t nlast = neb + snnz + 2 * (K.emax + 1) - 1;
       if (SCHEME == kLAK) {
-        const int32_t base1 = min(neb + 4 * e1 - bl1, nlast - (s1 < 0 ? 2 : 3));
-        const int32_t b = s1 < 0 ? 1 : 0;
+        const int32_t base1 = min(neb + 4 * c.e1 - c.bl1, nlast - (c.s1 < 0 ? 2 : 3));
+        const int32_t b = c.s1 < 0 ? 1 : 0;
         nbnd[base1] = b;
         nbnd[base1 + 1] = b;
         nbnd[base1 + 2] = 0;
-        if (s1 >= 0) nbnd[base1 + 3] = 0;
-      } else if (s1 < 0) {
-        nbnd[min(neb + 2 * snnz + bl1, nlast)] = 1;
+        if (c.s1 >= 0) nbnd[base1 + 3] = 0;
+      } else if (c.s1 < 0) {
+        nbnd[min(neb + 2 * snnz + c.bl1, nlast)] = 1;
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// refine: for every corner t of face f, v -> w the corner's out-half-edge
+// (edge e1, twin s1), u -> v the in-half-edge (edge e0, twin s0).
+//   refined quad t = (v, ep(e1), fp(f), ep(e0))   in the mesh's local numbering
+//   edge point e1  (written by the half-edge min(t, s1), or t on the boundary)
+//                  = (P_lo + P_hi + fp(face of lo->hi) + fp(face of hi->lo)) / 4,
+//                    (P_lo + P_hi) / 2 on the boundary
+// and for a level that feeds another, in closed form:
+//   twins          4t -> 4 next(s1) + 3, 4t+1 -> 4 next(t) + 2,
+//                  4t+2 -> 4 prev(t) + 1, 4t+3 -> 4 s0
+//   LAK edge ids   edge point e owns its edges to the two end points, then to
+//                  the face points, ranked by the smaller id: base(e) = 4e - bl(e)
+//   SLAK edge ids  vertices own one edge per corner, numbered by the corner's
+//                  M^T storage position; face points one per corner
+//                  (nnz + corner); boundary edge points one (2 nnz + bl(e))
+//   M^T rows       face point f: its corners 4t+2; edge point e: 4t+1,
+//                  4 next(t) + 3 of both half-edges, sorted (row 2 nnz + 4e - 2 bl(e))
+//   M^T positions  (SLAK) 4t keeps t's, 4t+2 -> nnz + t, 4t+1 / 4t+3 -> rank
+//                  in their edge point's row
+// bl(e) = boundary edges with a smaller id (0 on closed levels). One thread
+// per corner, so that each store instruction of a wave covers contiguous
+// memory; every load is issued before the first store.
+
+template <bool QUAD, int SCHEME>
+__global__ void __launch_bounds__(kBlock) k_refine_corner(RefineView R) {
+  const int64_t t64 = block_index() * blockDim.x + threadIdx.x;
+  if (t64 >= R.nnz) return;
+  CornerIn c;
+  c.t = (int32_t)t64;
+  corner_nbrs<QUAD>(R.fptr, R.cface, c.t, c.f, c.tn, c.tp);
+  const SegK K = seg_consts(R.seg, c.f);
+  if (R.ntptr && t64 == R.nnz - 1) R.ntptr[R.next_nv] = R.next_nnz;  // end of the next level's M^T
+  c.v = R.ids[c.t];
+  c.w = R.ids[c.tn];
+  c.u = R.ids[c.tp];
+  corner_topology<QUAD>(R, K, c, R.ci[c.t], R.ci[c.tp]);
+  const bool vlo = c.v < c.w;
+  const float4* __restrict__ fp = K.S->dst_verts + K.snv - K.fb;
+  if (c.own) {
+    const float4 Pv = R.P[c.v], Pw = R.P[c.w];
+    c.plo = vlo ? Pv : Pw;
+    c.phi = vlo ? Pw : Pv;
+    if (c.s1 >= 0) {
+      c.fa = fp[vlo ? c.f : c.g1];
+      c.fz = fp[vlo ? c.g1 : c.f];
+    }
+  }
+  corner_ranks<SCHEME>(R, K, c);
+  corner_store<QUAD, SCHEME>(R, K, c);
 }
 
 // vertex: one thread per vertex v with M^T row (t_i, u_i), i < n, sorted.
 //   interior  q = ((sum_i fp(face t_i) + sum_i P[u_i]) / n + P[v] (n - 2)) / n
 //   boundary  q = 3/4 P[v] + 1/8 (neighbours across boundary edges)
 // and for a level that feeds another: the vertex's refined M^T row (corners
-// 4 t_i, in-neighbours ep(e0(t_i))), SLAK ids of the edges it owns and their
-// boundary flags.
-constexpr int kVertexRegs = 4;  // row entries held in registers; longer rows loop
+// 4 t_i, in-neighbours ep(e0(t_i))) and, for SLAK, the boundary flags of the
+// edges it owns. The first kVertexRegs row entries and everything they gather
+// are loaded before any store; longer rows loop.
+constexpr int kVertexRegs = 4;
 
-template <bool QUAD, int SCHEME>
+template <bool QUAD, int SCHEME, bool NEXT, bool BND>
 __global__ void __launch_bounds__(kBlock) k_vertex(RefineView R) {
   const int32_t v = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (v >= R.nv) return;
@@ -508,11 +607,24 @@ __global__ void __launch_bounds__(kBlock) k_vertex(RefineView R) {
   float4* __restrict__ dst = S->dst_verts;
   const float4* __restrict__ fpts = dst + snv;  // face points, by mesh-local face
   const float4* __restrict__ P = R.P;
-  const int2* __restrict__ row = R.trow + R.tptr[v];
-  const int n = R.tptr[v + 1] - R.tptr[v];
+  const int2* __restrict__ ci = R.ci;
+  // NEXT: some mesh of the level continues (R.ntptr set); BND: the level has a
+  // boundary (R.blk set)
+  const bool next = NEXT && S->next_cb >= 0;
+  constexpr bool bnd_level = BND;
+  constexpr bool need_prev = NEXT || BND;  // (edge, twin) of the in-half-edge
+  constexpr bool need_own = BND;           // (edge, twin) of the out-half-edge
+  const int32_t p0 = R.tptr[v];
+  const int n = R.tptr[v + 1] - p0;
+  const int2* __restrict__ row = R.trow + p0;
   const float4 Pv = P[v];
   auto face = [&](int32_t t) __attribute__((always_inline)) { return (QUAD ? (t >> 2) : R.cface[t]) - fb; };
-  int2 e[kVertexRegs];
+  auto prevc = [&](int32_t t) __attribute__((always_inline)) {
+    int32_t f, tn, tp;
+    corner_nbrs<QUAD>(R.fptr, R.cface, t, f, tn, tp);
+    return tp;
+  };
+  int2 e[kVertexRegs], cp[kVertexRegs], co[kVertexRegs];
   float4 a[kVertexRegs], b[kVertexRegs];
 #pragma unroll
   for (int i = 0; i < kVertexRegs; ++i)
@@ -522,14 +634,16 @@ __global__ void __launch_bounds__(kBlock) k_vertex(RefineView R) {
     if (i < n) {
       a[i] = fpts[face(e[i].x)];
       b[i] = P[e[i].y];
+      if (need_prev) cp[i] = ci[prevc(e[i].x)];
+      if (need_own) co[i] = ci[e[i].x];
     }
   bool boundary = false;
-  if (R.blk)
-    for (int i = 0; i < n; ++i) {
-      int32_t f, tn, tp;
-      corner_nbrs<QUAD>(R.fptr, R.cface, row[i].x, f, tn, tp);
-      boundary |= R.ci[row[i].x].y < 0 || R.ci[tp].y < 0;
-    }
+  if (bnd_level) {
+#pragma unroll
+    for (int i = 0; i < kVertexRegs; ++i)
+      if (i < n) boundary |= co[i].y < 0 || cp[i].y < 0;
+    for (int i = kVertexRegs; i < n; ++i) boundary |= ci[row[i].x].y < 0 || ci[prevc(row[i].x)].y < 0;
+  }
   float4 q;
   if (!boundary) {
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -546,43 +660,55 @@ __global__ void __launch_bounds__(kBlock) k_vertex(RefineView R) {
   } else {
     q = f4mul(Pv, 0.75f);
     for (int i = 0; i < n; ++i) {
-      int32_t f, tn, tp;
-      corner_nbrs<QUAD>(R.fptr, R.cface, row[i].x, f, tn, tp);
-      if (R.ci[row[i].x].y < 0) q = f4add(q, f4mul(P[R.ids[tn]], 0.125f));
+      const int32_t t = row[i].x;
+      if (ci[t].y < 0) {
+        int32_t f, tn, tp;
+        corner_nbrs<QUAD>(R.fptr, R.cface, t, f, tn, tp);
+        q = f4add(q, f4mul(P[R.ids[tn]], 0.125f));
+      }
     }
-    for (int i = 0; i < n; ++i) {
-      int32_t f, tn, tp;
-      corner_nbrs<QUAD>(R.fptr, R.cface, row[i].x, f, tn, tp);
-      if (R.ci[tp].y < 0) q = f4add(q, f4mul(P[row[i].y], 0.125f));
-    }
+    for (int i = 0; i < n; ++i)
+      if (ci[prevc(row[i].x)].y < 0) q = f4add(q, f4mul(P[row[i].y], 0.125f));
   }
   dst[v - vb] = q;
-  if (S->next_cb < 0) return;
+  if (!next) return;
 
   const int32_t ncb = (int32_t)S->next_cb, nvb = (int32_t)S->next_vb;
   const int32_t nep = nvb + snv + S->nf;
   const int32_t emax = S->ne > 0 ? (int32_t)S->ne - 1 : 0;
-  const int32_t pos = R.tptr[v] - cb;  // mesh-local M^T storage position of the row
+  const int32_t pos = p0 - cb;  // mesh-local M^T storage position of the row
   R.ntptr[nvb + (v - vb)] = ncb + pos;
   int2* __restrict__ nrow = R.ntrow + ncb + pos;
-  int32_t* __restrict__ nci = reinterpret_cast<int32_t*>(R.nci);
-  for (int i = 0; i < n; ++i) {
-    const int32_t t = row[i].x;
-    int32_t f, tn, tp;
-    corner_nbrs<QUAD>(R.fptr, R.cface, t, f, tn, tp);
-    nrow[i] = make_int2(ncb + 4 * (t - cb), nep + clampi(R.ci[tp].x, emax));
-    if (SCHEME == kSLAK) {
-      const int32_t id = pos + i;
-      const int32_t tw = R.ci[t].y;
-      nci[2 * (int64_t)(ncb + 4 * (t - cb))] = id;
-      if (tw >= 0) {
-        int32_t g, twn, twp;
-        corner_nbrs<QUAD>(R.fptr, R.cface, tw, g, twn, twp);
-        nci[2 * (int64_t)(ncb + 4 * (twn - cb) + 3)] = id;
-      }
-      if (R.nbnd) R.nbnd[S->next_eb + id] = tw < 0 ? 1 : 0;
+  int32_t* __restrict__ nbnd = (SCHEME == kSLAK) ? R.nbnd : nullptr;
+  const int32_t neb = (int32_t)S->next_eb;
+#pragma unroll
+  for (int i = 0; i < kVertexRegs; ++i)
+    if (i < n) {
+      nrow[i] = make_int2(ncb + 4 * (e[i].x - cb), nep + clampi(cp[i].x, emax));
+      if (nbnd) nbnd[neb + pos + i] = co[i].y < 0 ? 1 : 0;
     }
+  for (int i = kVertexRegs; i < n; ++i) {
+    const int32_t t = row[i].x;
+    nrow[i] = make_int2(ncb + 4 * (t - cb), nep + clampi(ci[prevc(t)].x, emax));
+    if (nbnd) nbnd[neb + pos + i] = ci[t].y < 0 ? 1 : 0;
   }
+}
+
+// ---------------------------------------------------------------------------
+// LAK form of a final level: refined face r of a mesh has ptr[r] = 4r and
+// vals[4r .. 4r+3] = 1, 2, 3, 4 (createQuadColPtr / createQuadVals,
+// LAKCatmullClarkKernels.cuh:396-431); one thread per refined face (= corner
+// of the level's union).
+
+__global__ void __launch_bounds__(kBlock) k_lak_form(SegTable seg, int64_t nnz) {
+  const int64_t t = block_index() * blockDim.x + threadIdx.x;
+  if (t >= nnz) return;
+  const SegDesc* __restrict__ S = seg.d + seg_of(seg.cbase, seg.n, t);
+  if (!S->dst_ptr) return;
+  const int32_t lt = (int32_t)(t - S->cb);
+  store_nt(S->dst_ptr + lt, 4 * lt);
+  if (lt == (int32_t)S->nnz - 1) S->dst_ptr[S->nnz] = 4 * (int32_t)S->nnz;
+  if (S->dst_vals) store_nt(S->dst_vals + 4 * (int64_t)lt, 1, 2, 3, 4);
 }
 
 // ---------------------------------------------------------------------------
@@ -611,6 +737,85 @@ __device__ __forceinline__ int64_t block_exclusive_scan(int64_t x, int64_t* lds,
   __syncthreads();
   total = tot;
   return off + v - x;
+}
+
+// Inputs up to kScanSmall items: one block of 1024 threads in one launch
+// (instead of three). Chunks of 8192 items are loaded coalesced into
+// registers one chunk ahead, staged through LDS (padded by one word per 32 so
+// that each thread's 8 contiguous items sit in distinct banks), scanned per
+// thread and across the block, with a running carry.
+constexpr int kScanSmallThreads = 1024;
+constexpr int kScanSmallItems = 8;
+constexpr int kScanSmallChunk = kScanSmallThreads * kScanSmallItems;
+constexpr int64_t kScanSmall = 4 * (int64_t)kScanSmallChunk;
+
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
+
+__global__ void __launch_bounds__(kScanSmallThreads) k_scan_small(const int32_t* __restrict__ in, int64_t n,
+                                                                  int32_t* __restrict__ out, int* __restrict__ flags) {
+  __shared__ int32_t buf[kScanSmallChunk + kScanSmallChunk / 32];
+  __shared__ int64_t wsum[kScanSmallThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // chunk c + 1 is loaded (coalesced, into registers) while chunk c is scanned
+  int32_t cur[kScanSmallItems], nxt[kScanSmallItems];
+#pragma unroll
+  for (int k = 0; k < kScanSmallItems; ++k) {
+    const int64_t i = k * kScanSmallThreads + tid;
+    cur[k] = i < n ? in[i] : 0;
+  }
+  int64_t carry = 0;
+  for (int64_t base = 0; base < n; base += kScanSmallChunk) {
+#pragma unroll
+    for (int k = 0; k < kScanSmallItems; ++k) {
+      const int64_t i = base + kScanSmallChunk + k * kScanSmallThreads + tid;
+      nxt[k] = i < n ? in[i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kScanSmallItems; ++k) buf[lds_pad(k * kScanSmallThreads + tid)] = cur[k];
+    __syncthreads();
+    int32_t x[kScanSmallItems];
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanSmallItems; ++k) {
+      x[k] = buf[lds_pad(kScanSmallItems * tid + k)];
+      s += x[k];
+    }
+    int64_t v = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int64_t y = __shfl_up(v, d, 64);
+      if (lane >= d) v += y;
+    }
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    int64_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanSmallThreads / 64; ++w) {
+      const int64_t t = wsum[w];
+      off += (w < wave) ? t : 0;
+      tot += t;
+    }
+    int64_t run = carry + off + v - s;
+#pragma unroll
+    for (int k = 0; k < kScanSmallItems; ++k) {
+      buf[lds_pad(kScanSmallItems * tid + k)] = (int32_t)run;
+      run += x[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanSmallItems; ++k) {
+      const int i = k * kScanSmallThreads + tid;
+      if (base + i < n) out[base + i] = buf[lds_pad(i)];
+    }
+    carry += tot;
+#pragma unroll
+    for (int k = 0; k < kScanSmallItems; ++k) cur[k] = nxt[k];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (carry > 0x7fffffffll) atomicOr(flags, kFlagOverflow);
+    out[n] = (int32_t)carry;
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_scan_reduce(const int32_t* __restrict__ in, int64_t n,
@@ -686,9 +891,11 @@ void launch_fill_i32(int32_t* p, int64_t n, int32_t value, hipStream_t st) {
 }
 
 void launch_gather(const SegTable& seg, int32_t nf_total, bool use_vals, int32_t* ids_pos, int32_t* fptr,
-                   int32_t* cface, int* flags, hipStream_t st) {
-  if (nf_total <= 0) return;
-  k_gather<<<grid_for(nf_total), kBlock, 0, st>>>(seg, nf_total, use_vals ? 1 : 0, ids_pos, fptr, cface, flags);
+                   int32_t* cface, int* flags, int32_t* zero, int32_t nzero, hipStream_t st) {
+  const int32_t n = std::max(nf_total, nzero);
+  if (n <= 0) return;
+  k_gather<<<grid_for(n), kBlock, 0, st>>>(seg, nf_total, use_vals ? 1 : 0, ids_pos, fptr, cface, flags, zero,
+                                          nzero);
 }
 
 void launch_gather_verts(const SegTable& seg, int32_t nv_total, float4* P, hipStream_t st) {
@@ -697,27 +904,23 @@ void launch_gather_verts(const SegTable& seg, int32_t nv_total, float4* P, hipSt
 }
 
 void launch_row_hist(const int32_t* tptr, int32_t nv, const int32_t* fptr, int32_t nf, int32_t face_order,
-                     int32_t* hist, hipStream_t st) {
+                     int32_t* hist, const int64_t* vbase, int32_t nseg, const int32_t* eptr, int32_t* bounds,
+                     hipStream_t st) {
   int64_t g = (std::max<int64_t>(nv, nf) + kBlock - 1) / kBlock;
   if (g > 1024) g = 1024;
   if (g < 1) g = 1;
-  k_fill_i32<<<1, 64, 0, st>>>(hist, 2 * kHistBins, 0);
-  k_row_hist<<<(unsigned)g, kBlock, 0, st>>>(tptr, nv, fptr, nf, face_order, hist);
+  k_row_hist<<<(unsigned)g, kBlock, 0, st>>>(tptr, nv, fptr, nf, face_order, hist, vbase, nseg, eptr, bounds);
 }
 
 void launch_copy_probe(const float4* a, float4* b, int64_t n, hipStream_t st) {
   k_copy_probe<<<8192, kBlock, 0, st>>>(a, b, n);
 }
 
-void launch_seg_bounds(const int64_t* vbase, int32_t n, const int32_t* eptr, int32_t* out, hipStream_t st) {
-  k_seg_bounds<<<grid_for(n + 1), kBlock, 0, st>>>(vbase, n, eptr, out);
-}
-
 void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int32_t* fptr, const float4* P, int32_t nf,
-                  const SegTable& seg, int32_t* cnt, hipStream_t st) {
+                  const SegTable& seg, int32_t* cnt, const int* guard, hipStream_t st) {
   if (nf <= 0) return;
   unsigned g = grid_for(nf);
-#define SLAK_FACES(Q, A, B) k_faces<Q, A, B><<<g, kBlock, 0, st>>>(ids, fptr, P, nf, seg, cnt)
+#define SLAK_FACES(Q, A, B) k_faces<Q, A, B><<<g, kBlock, 0, st>>>(ids, fptr, P, nf, seg, cnt, guard)
   if (quad) {
     if (fp && count) SLAK_FACES(true, true, true);
     else if (fp) SLAK_FACES(true, true, false);
@@ -731,12 +934,12 @@ void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int3
 }
 
 void launch_transpose(bool quad, const int32_t* ids, const int32_t* fptr, int32_t nf, const int32_t* tptr,
-                      int32_t* cnt, int32_t* tids, hipStream_t st) {
+                      int32_t* cnt, int32_t* tids, const int* guard, hipStream_t st) {
   if (nf <= 0) return;
   if (quad)
-    k_transpose<true><<<grid_for(nf), kBlock, 0, st>>>(ids, fptr, nf, tptr, cnt, tids);
+    k_transpose<true><<<grid_for(nf), kBlock, 0, st>>>(ids, fptr, nf, tptr, cnt, tids, guard);
   else
-    k_transpose<false><<<grid_for(nf), kBlock, 0, st>>>(ids, fptr, nf, tptr, cnt, tids);
+    k_transpose<false><<<grid_for(nf), kBlock, 0, st>>>(ids, fptr, nf, tptr, cnt, tids, guard);
 }
 
 // cap: M^T row length held in registers (4 or 8); longer rows take the
@@ -784,13 +987,25 @@ void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st) {
 
 void launch_refine(bool quad, int scheme, const RefineView& R, hipStream_t st) {
   if (R.nnz <= 0) return;
-  unsigned g = grid_for(R.nnz);
+  const unsigned g = grid_for(R.nnz);
   if (quad) {
-    if (scheme == kLAK) k_refine<true, kLAK><<<g, kBlock, 0, st>>>(R);
-    else k_refine<true, kSLAK><<<g, kBlock, 0, st>>>(R);
+    if (scheme == kLAK) k_refine_corner<true, kLAK><<<g, kBlock, 0, st>>>(R);
+    else k_refine_corner<true, kSLAK><<<g, kBlock, 0, st>>>(R);
   } else {
-    if (scheme == kLAK) k_refine<false, kLAK><<<g, kBlock, 0, st>>>(R);
-    else k_refine<false, kSLAK><<<g, kBlock, 0, st>>>(R);
+    if (scheme == kLAK) k_refine_corner<false, kLAK><<<g, kBlock, 0, st>>>(R);
+    else k_refine_corner<false, kSLAK><<<g, kBlock, 0, st>>>(R);
+  }
+}
+
+template <bool QUAD, int SCHEME>
+void launch_vertex_t(const RefineView& R, unsigned g, hipStream_t st) {
+  const bool next = R.ntptr != nullptr, bnd = R.blk != nullptr;
+  if (next) {
+    if (bnd) k_vertex<QUAD, SCHEME, true, true><<<g, kBlock, 0, st>>>(R);
+    else k_vertex<QUAD, SCHEME, true, false><<<g, kBlock, 0, st>>>(R);
+  } else {
+    if (bnd) k_vertex<QUAD, SCHEME, false, true><<<g, kBlock, 0, st>>>(R);
+    else k_vertex<QUAD, SCHEME, false, false><<<g, kBlock, 0, st>>>(R);
   }
 }
 
@@ -798,22 +1013,27 @@ void launch_vertex(bool quad, int scheme, const RefineView& R, hipStream_t st) {
   if (R.nv <= 0) return;
   unsigned g = grid_for(R.nv);
   if (quad) {
-    if (scheme == kLAK) k_vertex<true, kLAK><<<g, kBlock, 0, st>>>(R);
-    else k_vertex<true, kSLAK><<<g, kBlock, 0, st>>>(R);
+    if (scheme == kLAK) launch_vertex_t<true, kLAK>(R, g, st);
+    else launch_vertex_t<true, kSLAK>(R, g, st);
   } else {
-    if (scheme == kLAK) k_vertex<false, kLAK><<<g, kBlock, 0, st>>>(R);
-    else k_vertex<false, kSLAK><<<g, kBlock, 0, st>>>(R);
+    if (scheme == kLAK) launch_vertex_t<false, kLAK>(R, g, st);
+    else launch_vertex_t<false, kSLAK>(R, g, st);
   }
+}
+
+void launch_lak_form(const SegTable& seg, int64_t nnz, hipStream_t st) {
+  if (nnz <= 0) return;
+  k_lak_form<<<grid_for(nnz), kBlock, 0, st>>>(seg, nnz);
 }
 
 int64_t scan_partials_len(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
 void launch_scan(const int32_t* in, int32_t* out, int64_t n, int64_t* partial, int* flags, hipStream_t st) {
-  int64_t ntiles = (n + kScanTile - 1) / kScanTile;
-  if (ntiles == 0) {
-    k_fill_i32<<<1, 64, 0, st>>>(out, 1, 0);
+  if (n <= kScanSmall) {
+    k_scan_small<<<1, kScanSmallThreads, 0, st>>>(in, n, out, flags);
     return;
   }
+  int64_t ntiles = (n + kScanTile - 1) / kScanTile;
   k_scan_reduce<<<(unsigned)ntiles, kBlock, 0, st>>>(in, n, partial);
   k_scan_partials<<<1, kBlock, 0, st>>>(partial, ntiles, out, n, flags);
   k_scan_down<<<(unsigned)ntiles, kBlock, 0, st>>>(in, n, partial, out);

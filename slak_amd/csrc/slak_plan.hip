@@ -107,18 +107,37 @@ int Plan::readback(const int32_t* src, int count, int64_t* dst) {
   return p32[0];
 }
 
-void Plan::upload_tables(LevelPlan& L) {
-  size_t n = L.segs.size();
-  if (!L.d_segs) {
-    L.d_segs = arena_.get_n<SegDesc>((int64_t)n);
-    L.d_vbase = arena_.get_n<int64_t>((int64_t)n + 1);
-    L.d_fbase = arena_.get_n<int64_t>((int64_t)n + 1);
-    L.d_mflags = arena_.get_n<int>((int64_t)n);
+// Uploads the segment tables (SegDesc, vertex, face and corner bases) of levels
+// [l0, l1) in one copy from a plan-owned staging buffer (one per call site,
+// so a pending copy never sees its source rewritten).
+void Plan::upload_tables(int l0, int l1, std::vector<char>& staging) {
+  auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  size_t total = 0;
+  std::vector<size_t> offs;
+  for (int l = l0; l < l1; ++l) {
+    const size_t n = levels_[l].segs.size();
+    offs.push_back(total);
+    total = align(total + n * sizeof(SegDesc)) + align(3 * (n + 1) * sizeof(int64_t));
   }
-  SLAK_HIP_CHECK(hipMemcpyAsync(L.d_segs, L.segs.data(), n * sizeof(SegDesc), hipMemcpyHostToDevice, st_));
-  SLAK_HIP_CHECK(hipMemcpyAsync(L.d_vbase, L.vbase.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st_));
-  SLAK_HIP_CHECK(hipMemcpyAsync(L.d_fbase, L.fbase.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st_));
-  // the host vectors live as long as the plan, past every copy's completion
+  if (total == 0) return;
+  staging.assign(total, 0);
+  char* d = arena_.get_n<char>((int64_t)total);
+  for (int l = l0; l < l1; ++l) {
+    LevelPlan& L = levels_[l];
+    const size_t n = L.segs.size();
+    size_t o = offs[l - l0];
+    std::memcpy(staging.data() + o, L.segs.data(), n * sizeof(SegDesc));
+    L.d_segs = reinterpret_cast<SegDesc*>(d + o);
+    o = align(o + n * sizeof(SegDesc));
+    std::memcpy(staging.data() + o, L.vbase.data(), (n + 1) * sizeof(int64_t));
+    L.d_vbase = reinterpret_cast<int64_t*>(d + o);
+    std::memcpy(staging.data() + o + (n + 1) * sizeof(int64_t), L.fbase.data(), (n + 1) * sizeof(int64_t));
+    L.d_fbase = reinterpret_cast<int64_t*>(d + o) + (n + 1);
+    std::memcpy(staging.data() + o + 2 * (n + 1) * sizeof(int64_t), L.cbase.data(), (n + 1) * sizeof(int64_t));
+    L.d_cbase = reinterpret_cast<int64_t*>(d + o) + 2 * (n + 1);
+    if (!L.d_mflags) L.d_mflags = arena_.get_n<int>((int64_t)std::max<size_t>(n, 1));
+  }
+  SLAK_HIP_CHECK(hipMemcpyAsync(d, staging.data(), total, hipMemcpyHostToDevice, st_));
 }
 
 std::string Plan::describe_failure(int fl) const {
@@ -165,12 +184,14 @@ void Plan::plan_first_level() {
     L.segs.push_back(S);
     L.vbase.push_back(vb);
     L.fbase.push_back(fb);
+    L.cbase.push_back(cb);
     vb += S.nv;
     fb += S.nf;
     cb += S.nnz;
   }
   L.vbase.push_back(vb);
   L.fbase.push_back(fb);
+  L.cbase.push_back(cb);
   L.nv = vb;
   L.nf = fb;
   L.nnz = cb;
@@ -191,11 +212,17 @@ void Plan::plan_first_level() {
     P1_ = arena_.get_n<float4>(L.nv);
     L.P = P1_;
   }
-  L.cnt = arena_.get_n<int32_t>(L.nv);
+  // [row-length histograms | per-mesh edge bounds | valence counters], zeroed
+  // by the gather; the first two are read back together
+  const int64_t nseg = (int64_t)L.segs.size();
+  d_counters_ = arena_.get_n<int32_t>(2 * kRowHistBins + nseg + 1 + L.nv);
+  n_counters_ = 2 * kRowHistBins + nseg + 1 + L.nv;
+  L.cnt = d_counters_ + 2 * kRowHistBins + nseg + 1;
   L.tptr = arena_.get_n<int32_t>(L.nv + 1);
   L.tids = arena_.get_n<int32_t>(L.nnz);
   L.trow = arena_.get_n<int2>(L.nnz);
   L.ci = arena_.get_n<int2>(L.nnz);
+  if (scheme_ == kSLAK) L.mpos = arena_.get_n<int32_t>(L.nnz);
   L.ecnt = arena_.get_n<int32_t>(L.nv);
   L.eptr = arena_.get_n<int32_t>(L.nv + 1);
 }
@@ -206,12 +233,8 @@ void Plan::plan_first_level() {
 void Plan::record_analysis(LevelPlan& L) {
   const double poly_b = L.quad ? 0.0 : 4.0 * (L.nf + 1) + 4.0 * L.nnz;
   {
-    ProfScope ps(st_, "fill", 4.0 * L.nv);
-    launch_fill_i32(L.cnt, L.nv, 0, st_);
-  }
-  {
     ProfScope ps(st_, L.quad ? "count_quad" : "count_poly", 4.0 * L.nnz + 8.0 * L.nv + (L.quad ? 0.0 : 4.0 * L.nf));
-    launch_faces(L.quad, false, true, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), L.cnt, st_);
+    launch_faces(L.quad, false, true, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), L.cnt, d_flags_, st_);
   }
   {
     ProfScope ps(st_, "scan", 8.0 * L.nv + 4);
@@ -220,7 +243,7 @@ void Plan::record_analysis(LevelPlan& L) {
   {
     ProfScope ps(st_, L.quad ? "transpose_quad" : "transpose_poly",
                  8.0 * L.nnz + 12.0 * L.nv + (L.quad ? 0.0 : 4.0 * L.nf));
-    launch_transpose(L.quad, L.ids, L.fptr, (int32_t)L.nf, L.tptr, L.cnt, L.tids, st_);
+    launch_transpose(L.quad, L.ids, L.fptr, (int32_t)L.nf, L.tptr, L.cnt, L.tids, d_flags_, st_);
   }
   LevelView V{};
   V.ids = L.ids;
@@ -233,6 +256,7 @@ void Plan::record_analysis(LevelPlan& L) {
   V.tptr = L.tptr;
   V.tids = L.tids;
   V.trow = L.trow;
+  V.mpos = L.mpos;
   V.ecnt = L.ecnt;
   V.flags = d_flags_;
   V.seg = L.table();
@@ -275,7 +299,7 @@ void Plan::prepare() {
   plan_first_level();
   LevelPlan& L1 = levels_[0];
   const int nseg = (int)L1.segs.size();
-  upload_tables(L1);
+  upload_tables(0, 1, staging_first_);
   int64_t need = scan_partials_len(L1.nv);
   d_partial_ = arena_.get_n<int64_t>(need);
   partial_len_ = need;
@@ -286,7 +310,8 @@ void Plan::prepare() {
   {
     ProfScope ps(st_, "gather", 4.0 * (L1.nf + 1) + 12.0 * L1.nnz + (L1.quad ? 0.0 : 8.0 * L1.nnz + 4.0 * L1.nf));
     launch_gather(L1.table(), (int32_t)L1.nf, scheme_ == kLAK, const_cast<int32_t*>(L1.ids),
-                  const_cast<int32_t*>(L1.fptr), const_cast<int32_t*>(L1.cface), d_flags_, st_);
+                  const_cast<int32_t*>(L1.fptr), const_cast<int32_t*>(L1.cface), d_flags_, d_counters_,
+                  (int32_t)n_counters_, st_);
   }
   if (P1_) launch_gather_verts(L1.table(), (int32_t)L1.nv, P1_, st_);
   auto fail_if = [&](int fl) {
@@ -301,28 +326,31 @@ void Plan::prepare() {
     all_trusted &= jobs_[L1.jobs[k]].trusted;
     all_known &= jobs_[L1.jobs[k]].known_ne >= 0;
   }
-  if (!all_trusted) fail_if(readback(nullptr, 0, nullptr));
-
   // 2. first-level analysis: M^T, owned-edge counts, per-mesh edge totals
+  // (its kernels do nothing on an input the gather flagged; the flags come
+  // back with the edge counts in one round trip)
   record_analysis(L1);
-  d_bounds_ = arena_.get_n<int32_t>(nseg + 1 + 2 * kRowHistBins);
-  launch_seg_bounds(L1.d_vbase, nseg, L1.eptr, d_bounds_, st_);
-  launch_row_hist(L1.tptr, (int32_t)L1.nv, L1.fptr, (int32_t)L1.nf, L1.quad ? 4 : 0, d_bounds_ + nseg + 1, st_);
-  std::vector<int64_t> bounds(nseg + 1 + 2 * kRowHistBins, 0);
+  int32_t* d_hist = d_counters_;
+  int32_t* d_bounds = d_counters_ + 2 * kRowHistBins;
+  launch_row_hist(L1.tptr, (int32_t)L1.nv, L1.fptr, (int32_t)L1.nf, L1.quad ? 4 : 0, d_hist, L1.d_vbase, nseg,
+                  L1.eptr, d_bounds, st_);
+  std::vector<int64_t> hb(2 * kRowHistBins + nseg + 1, 0);
   bool have_hist = false;
   if (!all_known) {
-    fail_if(readback(d_bounds_, nseg + 1 + 2 * kRowHistBins, bounds.data()));
+    fail_if(readback(d_hist, 2 * kRowHistBins + nseg + 1, hb.data()));
     have_hist = true;
   } else {
-    for (int k = 0; k < nseg; ++k) bounds[k + 1] = bounds[k] + jobs_[L1.jobs[k]].known_ne;
+    if (!all_trusted) fail_if(readback(nullptr, 0, nullptr));
+    for (int k = 0; k < nseg; ++k) hb[2 * kRowHistBins + k + 1] = hb[2 * kRowHistBins + k] + jobs_[L1.jobs[k]].known_ne;
   }
+  const int64_t* bounds = hb.data() + 2 * kRowHistBins;
   // row-length statistics of every level, predicted from the first level:
   // original vertices keep their valence, face points take the face order,
   // edge points have 4 (2 on the boundary), later face points 4.
   int64_t vlong = 0, flong = 0;
   int vmax = 0, fmax = 0;
   if (have_hist) {
-    const int64_t* hv = bounds.data() + nseg + 1;
+    const int64_t* hv = hb.data();
     const int64_t* hf = hv + kRowHistBins;
     for (int b = 0; b < kRowHistBins; ++b) {
       if (b > 4) {
@@ -422,6 +450,7 @@ void Plan::prepare() {
       L.segs.clear();
       L.vbase.clear();
       L.fbase.clear();
+      L.cbase.clear();
       int64_t vb = 0, fb = 0, cb = 0, eb = 0;
       for (int j : prev.jobs) {
         MeshJob& J = jobs_[j];
@@ -438,6 +467,7 @@ void Plan::prepare() {
         L.segs.push_back(S);
         L.vbase.push_back(vb);
         L.fbase.push_back(fb);
+        L.cbase.push_back(cb);
         vb += S.nv;
         fb += S.nf;
         cb += S.nnz;
@@ -445,6 +475,7 @@ void Plan::prepare() {
       }
       L.vbase.push_back(vb);
       L.fbase.push_back(fb);
+      L.cbase.push_back(cb);
       L.nv = vb;
       L.nf = fb;
       L.nnz = cb;
@@ -475,6 +506,7 @@ void Plan::prepare() {
         S.dst_ids = J.out_ids;
         S.dst_ptr = J.out_ptr;
         S.dst_vals = J.out_vals;
+        L.lak_form |= J.out_ptr != nullptr;
         S.id_offset = 0;
         S.next_vb = -1;
         S.next_cb = -1;
@@ -509,14 +541,21 @@ void Plan::prepare() {
     int32_t* mt_ptr[2] = {arena_.get_n<int32_t>(max_nv + 1), Lmax >= 3 ? arena_.get_n<int32_t>(max_nv + 1) : nullptr};
     int2* mt_row[2] = {arena_.get_n<int2>(max_nnz), Lmax >= 3 ? arena_.get_n<int2>(max_nnz) : nullptr};
     int2* mt_ci[2] = {arena_.get_n<int2>(max_nnz), Lmax >= 3 ? arena_.get_n<int2>(max_nnz) : nullptr};
+    int32_t* mt_pos[2] = {nullptr, nullptr};
+    if (scheme_ == kSLAK) {
+      mt_pos[0] = arena_.get_n<int32_t>(max_nnz);
+      if (Lmax >= 3) mt_pos[1] = arena_.get_n<int32_t>(max_nnz);
+    }
     for (int l = 2; l <= Lmax; ++l) {
       LevelPlan& L = levels_[l - 1];
       L.tptr = mt_ptr[l % 2];
       L.trow = mt_row[l % 2];
       L.ci = mt_ci[l % 2];
+      L.mpos = mt_pos[l % 2];
       levels_[l - 2].ntptr = L.tptr;
       levels_[l - 2].ntrow = L.trow;
       levels_[l - 2].nci = L.ci;
+      levels_[l - 2].nmpos = L.mpos;
     }
   }
   int64_t max_ne = 0;
@@ -548,7 +587,7 @@ void Plan::prepare() {
     L.max_row = l == 1 ? vmax : std::max(std::max(vmax, fmax), 4);
     L.cap = autotune_cap(L.quad, L.nv, L.long_rows, L.max_row);  // used by the first level's row kernels
   }
-  for (auto& L : levels_) upload_tables(L);
+  upload_tables(0, (int)levels_.size(), staging_all_);
   prepared_ = true;
 }
 
@@ -584,7 +623,7 @@ void Plan::record_level(int l) {
   {
     ProfScope ps(st_, L.quad ? "facepoints_quad" : "facepoints_poly",
                  4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + (L.quad ? 0.0 : 4.0 * (L.nf + 1)));
-    launch_faces(L.quad, true, false, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), nullptr, st_);
+    launch_faces(L.quad, true, false, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), nullptr, nullptr, st_);
   }
   RefineView R{};
   R.ids = L.ids;
@@ -598,25 +637,23 @@ void Plan::record_level(int l) {
   R.trow = L.trow;
   R.ci = L.ci;
   R.blk = L.blk;
+  R.mpos = L.mpos;
   R.seg = L.table();
   R.ntptr = L.ntptr;
   R.ntrow = L.ntrow;
   R.nci = L.nci;
   R.nbnd = L.nbnd;
-  if (L.ntptr) {  // last offset of the next level's M^T
-    launch_fill_i32(L.ntptr + L.next_nv, 1, (int32_t)L.next_nnz, st_);
-  }
+  R.nmpos = L.nmpos;
+  R.next_nv = (int32_t)L.next_nv;
+  R.next_nnz = (int32_t)L.next_nnz;
   const bool next = L.ntptr != nullptr;
   {
-    double nnz_final = 0;
-    for (auto& S : L.segs)
-      if (S.dst_ptr) nnz_final += (double)S.nnz;
     // reads ids, (edge, twin) pairs, positions and face points; writes refined
-    // quads, edge points, the final level's LAK form and the next level's
-    // (edge, twin) pairs, face- and edge-point rows of M^T
+    // quads, edge points and the next level's (edge, twin) pairs, face- and
+    // edge-point rows of M^T
     ProfScope ps(st_, L.quad ? "refine_quad" : "refine_poly",
                  4.0 * L.nnz + 8.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 16.0 * L.nnz + 16.0 * ne +
-                     20.0 * nnz_final + (L.blk ? 4.0 * ne : 0.0) + poly_b +
+                     (L.blk ? 4.0 * ne : 0.0) + poly_b +
                      (next ? 32.0 * L.nnz + 24.0 * L.nnz + 4.0 * (L.nf + ne) + (L.nbnd ? 16.0 * L.nnz : 0.0) : 0.0));
     launch_refine(L.quad, scheme_, R, st_);
   }
@@ -630,16 +667,41 @@ void Plan::record_level(int l) {
   }
 }
 
+// The LAK form of a final level (ptr[t] = 4t, vals cyclic 1..4,
+// createQuadColPtr / createQuadVals, LAKCatmullClarkImpl.cu:553-565) depends
+// only on sizes: it is written on an auxiliary stream while the chain runs,
+// mostly during the small first levels that leave the GPU idle.
+void Plan::launch_lak_forms() {
+  bool any = false;
+  for (auto& L : levels_) any |= L.lak_form;
+  if (!any) return;
+  AuxStream& aux = aux_stream();
+  SLAK_HIP_CHECK(hipEventRecord(aux.fork, st_));
+  SLAK_HIP_CHECK(hipStreamWaitEvent(aux.stream, aux.fork, 0));
+  for (auto& L : levels_) {
+    if (!L.lak_form) continue;
+    double nnz_final = 0;
+    for (auto& S : L.segs)
+      if (S.dst_ptr) nnz_final += (double)S.nnz;
+    ProfScope ps(aux.stream, "lak_form", 20.0 * nnz_final + 4.0);
+    launch_lak_form(L.table(), L.nnz, aux.stream);
+  }
+  SLAK_HIP_CHECK(hipEventRecord(aux.join, aux.stream));
+  lak_forms_pending_ = true;
+}
+
 void Plan::launch(bool include_analysis) {
   if (levels_.empty()) return;
   LevelPlan& L1 = levels_[0];
+  launch_lak_forms();
   if (include_analysis) {
     launch_fill_i32(d_flags_, 1, 0, st_);
     launch_fill_i32(L1.d_mflags, (int64_t)L1.segs.size(), 0, st_);
     {
       ProfScope ps(st_, "gather", 4.0 * (L1.nf + 1) + 12.0 * L1.nnz + (L1.quad ? 0.0 : 8.0 * L1.nnz + 4.0 * L1.nf));
       launch_gather(L1.table(), (int32_t)L1.nf, scheme_ == kLAK, const_cast<int32_t*>(L1.ids),
-                    const_cast<int32_t*>(L1.fptr), const_cast<int32_t*>(L1.cface), d_flags_, st_);
+                    const_cast<int32_t*>(L1.fptr), const_cast<int32_t*>(L1.cface), d_flags_, d_counters_,
+                    (int32_t)n_counters_, st_);
     }
     if (P1_) {
       ProfScope ps(st_, "gather_verts", 32.0 * L1.nv);
@@ -648,6 +710,10 @@ void Plan::launch(bool include_analysis) {
     record_analysis(L1);
   }
   for (int l = 1; l <= (int)levels_.size(); ++l) record_level(l);
+  if (lak_forms_pending_) {
+    SLAK_HIP_CHECK(hipStreamWaitEvent(st_, aux_stream().join, 0));
+    lak_forms_pending_ = false;
+  }
 }
 
 void Plan::run_graph() {

@@ -56,20 +56,24 @@ struct LevelPlan {
   int2* trow = nullptr;
   int2* ci = nullptr;
   int32_t *bnd = nullptr, *blk = nullptr;
+  int32_t* mpos = nullptr;  // M^T storage position per corner (SLAK)
   // the next level's structure, written by this level's refine and vertex kernels
   int32_t* ntptr = nullptr;
   int2* ntrow = nullptr;
   int2* nci = nullptr;
   int32_t* nbnd = nullptr;
+  int32_t* nmpos = nullptr;
   int64_t next_nv = 0, next_nnz = 0;  // next level's union sizes
   std::vector<int> jobs;  // segment -> job
   std::vector<SegDesc> segs;
-  std::vector<int64_t> vbase, fbase;
+  std::vector<int64_t> vbase, fbase, cbase;
+  bool lak_form = false;  // some mesh finishes here with the LAK form (ptr / vals)
   SegDesc* d_segs = nullptr;
   int64_t* d_vbase = nullptr;
   int64_t* d_fbase = nullptr;
+  int64_t* d_cbase = nullptr;
   int* d_mflags = nullptr;
-  SegTable table() const { return SegTable{d_segs, d_vbase, d_fbase, (int32_t)segs.size(), d_mflags}; }
+  SegTable table() const { return SegTable{d_segs, d_vbase, d_fbase, (int32_t)segs.size(), d_mflags, d_cbase}; }
 };
 
 enum class OutputMode {
@@ -84,8 +88,9 @@ class Plan {
   Plan(const Plan&) = delete;
   Plan& operator=(const Plan&) = delete;
 
-  // Validates the inputs and runs the first level's analysis (two blocking
-  // round trips), then sizes and allocates everything. Throws on bad input.
+  // Validates the inputs and runs the first level's analysis (one blocking
+  // round trip, none for trusted meshes of known edge count), then sizes and
+  // allocates everything. Throws on bad input.
   void prepare();
   // Launches the chain on the plan's stream. With include_analysis the first
   // level is recomputed from the caller's meshes (used for graph replays).
@@ -111,7 +116,7 @@ class Plan {
   void plan_first_level();
   void record_analysis(LevelPlan& L);
   void record_level(int l);
-  void upload_tables(LevelPlan& L);
+  void upload_tables(int l0, int l1, std::vector<char>& staging);
   int readback(const int32_t* src, int count, int64_t* dst);
 
   int scheme_;
@@ -123,7 +128,9 @@ class Plan {
   int* d_flags_ = nullptr;
   int64_t* d_partial_ = nullptr;
   int64_t partial_len_ = 0;
-  int32_t* d_bounds_ = nullptr;
+  int32_t* d_counters_ = nullptr;  // first level: histograms | edge bounds | valence counters
+  int64_t n_counters_ = 0;
+  std::vector<char> staging_first_, staging_all_;  // segment table uploads
   // first-level union gather (multi-mesh)
   float4* P1_ = nullptr;
   bool prepared_ = false;
@@ -134,6 +141,8 @@ class Plan {
   hipStream_t capture_st_ = nullptr;  // stream the graph is captured on and replayed
   hipStream_t owned_st_ = nullptr;    // created when the caller's stream is the default one
   hipEvent_t order_ev_ = nullptr;
+  bool lak_forms_pending_ = false;
+  void launch_lak_forms();
   std::vector<void*> pooled_outputs_;
 };
 

@@ -3,7 +3,8 @@ vertex kernels apply (slak_amd/csrc/slak_kernels.hip, k_refine / k_vertex).
 
 TEST INFRASTRUCTURE: pins the closed forms by which level l writes the
 structure of level l+1 (edge ids of every corner's outgoing half-edge, twin
-corners, sorted M^T rows with in-neighbours, boundary flags per edge) against
+corners, sorted M^T rows with in-neighbours, each corner's M^T storage
+position, boundary flags per edge) against
 the same structure computed by brute force on the refined mesh, whose faces
 come from the oracle. The brute force (``structure_of``) follows the
 canonical numberings the reference's kernels produce when their atomics are
@@ -32,6 +33,7 @@ class Structure:
     bnd: np.ndarray          # ne: boundary flag by edge id
     tptr: np.ndarray         # nv + 1
     rows: List[Tuple[int, int]]  # nnz (corner, in-neighbour), rows sorted by corner
+    mpos: np.ndarray         # nnz: position of the corner in the M^T storage
 
 
 def _corner_tables(ptr: np.ndarray):
@@ -85,11 +87,14 @@ def structure_of(ptr: np.ndarray, ids: np.ndarray, nv: int, scheme: int) -> Stru
             bnd[eout[t]] = 1
     tptr = np.zeros(nv + 1, dtype=np.int64)
     rows = []
+    mpos = np.zeros(nnz, dtype=np.int64)
     for v in range(nv):
         r = sorted(rows_by_v[v])
         tptr[v + 1] = tptr[v] + len(r)
+        for i, t in enumerate(r):
+            mpos[t] = tptr[v] + i
         rows.extend((t, int(ids[prevc[t]])) for t in r)
-    return Structure(ptr, ids, nv, eout, twin, bnd, tptr, rows)
+    return Structure(ptr, ids, nv, eout, twin, bnd, tptr, rows, mpos)
 
 
 def refine_structure(S: Structure, scheme: int) -> Structure:
@@ -108,6 +113,7 @@ def refine_structure(S: Structure, scheme: int) -> Structure:
     ntptr = np.full(nv2 + 1, UNSET, dtype=np.int64)
     nbnd = np.full(ne2, UNSET, dtype=np.int64)
     nids = np.zeros(nnz2, dtype=np.int64)
+    nmpos = np.full(nnz2, UNSET, dtype=np.int64)
     # ---- refine: one step per corner t
     for t in range(nnz):
         f, tn, tp = int(cface[t]), int(nextc[t]), int(prevc[t])
@@ -125,9 +131,20 @@ def refine_structure(S: Structure, scheme: int) -> Structure:
             b1, b0 = 4 * e1 - bl1, 4 * e0 - bl0
             eo = [b1 + (v > w), b1 + 2 + (s1 >= 0 and f > g1), b0 + 2 + (s0 >= 0 and f > g0), b0 + (v > u)]
         else:
-            eo = [UNSET, nnz + tn, nnz + t, 2 * nnz + bl0 if s0 < 0 else UNSET]
+            # vertex-owned edges take the owning corner's M^T position
+            eo = [int(S.mpos[t]), nnz + tn, nnz + t, 2 * nnz + bl0 if s0 < 0 else int(S.mpos[s0])]
         for k in range(4):
             nci[4 * t + k] = (eo[k], tw[k])
+        # M^T positions of the refined corners: vertex rows keep their order,
+        # face point rows hold corner 2 in corner order, edge point rows are
+        # sorted (rank among the row's 2 or 4 corners)
+        r1 = [4 * tn + 3] + ([4 * s1 + 1, 4 * n1 + 3] if s1 >= 0 else [])
+        ns0 = int(nextc[s0]) if s0 >= 0 else 0
+        r0 = [4 * tp + 1] + ([4 * s0 + 1, 4 * ns0 + 3] if s0 >= 0 else [])
+        nmpos[4 * t] = S.mpos[t]
+        nmpos[4 * t + 1] = 2 * nnz + 4 * e1 - 2 * bl1 + sum(x < 4 * t + 1 for x in r1)
+        nmpos[4 * t + 2] = nnz + t
+        nmpos[4 * t + 3] = 2 * nnz + 4 * e0 - 2 * bl0 + sum(x < 4 * t + 3 for x in r0)
         ntrow[nnz + t] = (4 * t + 2, nv + nf + e1)
         if t == int(ptr[f]):
             ntptr[nv + f] = nnz + t
@@ -155,17 +172,13 @@ def refine_structure(S: Structure, scheme: int) -> Structure:
             t, _ = S.rows[pos + i]
             ntrow[pos + i] = (4 * t, nv + nf + int(S.eout[prevc[t]]))
             if scheme == SLAK:
-                nci[4 * t, 0] = pos + i
-                tw = int(S.twin[t])
-                if tw >= 0:
-                    nci[4 * int(nextc[tw]) + 3, 0] = pos + i
-                nbnd[pos + i] = 1 if tw < 0 else 0
+                nbnd[pos + i] = 1 if int(S.twin[t]) < 0 else 0
     ntptr[nv2] = nnz2
     assert (nci != UNSET).all() and all(r is not None for r in ntrow)
-    assert (ntptr != UNSET).all() and (nbnd != UNSET).all()
+    assert (ntptr != UNSET).all() and (nbnd != UNSET).all() and (nmpos != UNSET).all()
     assert int(nbnd.sum()) == 2 * n_ext
     return Structure(np.arange(nnz + 1, dtype=np.int64) * 4, nids, nv2, nci[:, 0], nci[:, 1], nbnd, ntptr,
-                     [tuple(map(int, r)) for r in ntrow])
+                     [tuple(map(int, r)) for r in ntrow], nmpos)
 
 
 def b1_of(e: int, bl: int) -> int:

@@ -152,8 +152,10 @@ def test_spgemm_operator_matches_oracle(oracle, make):
     assert r.nedges == d["nedges"] and r.nextern == d["nextern"]
 
 
-@pytest.mark.parametrize("n", [0, 1, 7, 2047, 2048, 2049, 100000, 3_000_001])
+@pytest.mark.parametrize("n", [0, 1, 7, 2047, 2048, 2049, 8191, 8192, 8193, 32767, 32768, 32769, 100000, 3_000_001])
 def test_scan_operator(n):
+    """Exclusive scan with total; sizes around the one-block chunk (8192) and
+    the one-block / three-kernel threshold (32768)."""
     import ctypes
     from slak_amd.mesh import DeviceBuffer, to_host
     rng = np.random.default_rng(n)

@@ -26,6 +26,7 @@ def _same(a: M.Structure, b: M.Structure):
     np.testing.assert_array_equal(a.bnd, b.bnd)
     np.testing.assert_array_equal(a.tptr, b.tptr)
     assert a.rows == b.rows
+    np.testing.assert_array_equal(a.mpos, b.mpos)
 
 
 @pytest.mark.parametrize("name,make", MESHES, ids=[m[0] for m in MESHES])
