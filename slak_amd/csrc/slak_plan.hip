@@ -693,7 +693,7 @@ void Plan::launch_lak_forms() {
 void Plan::launch(bool include_analysis) {
   if (levels_.empty()) return;
   LevelPlan& L1 = levels_[0];
-  launch_lak_forms();
+  if (!lak_forms_in_place_) launch_lak_forms();
   if (include_analysis) {
     launch_fill_i32(d_flags_, 1, 0, st_);
     launch_fill_i32(L1.d_mflags, (int64_t)L1.segs.size(), 0, st_);
@@ -744,6 +744,13 @@ void Plan::run_graph() {
     // event-pair timing stays out of the captured graph
     const bool prof = Profiler::get().enabled();
     Profiler::get().set_enabled(false);
+    // the LAK forms are constant: written once here, outside the graph
+    launch_lak_forms();
+    if (lak_forms_pending_) {
+      SLAK_HIP_CHECK(hipStreamWaitEvent(st_, aux_stream().join, 0));
+      lak_forms_pending_ = false;
+    }
+    lak_forms_in_place_ = true;
     SLAK_HIP_CHECK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
     try {
       launch(true);

@@ -142,6 +142,7 @@ class Plan {
   hipStream_t owned_st_ = nullptr;    // created when the caller's stream is the default one
   hipEvent_t order_ev_ = nullptr;
   bool lak_forms_pending_ = false;
+  bool lak_forms_in_place_ = false;  // written before a graph capture; replays keep them
   void launch_lak_forms();
   std::vector<void*> pooled_outputs_;
 };

@@ -90,6 +90,10 @@ def test_graph_replays_match_eager_and_track_new_positions():
                     got = fetch(b.result(i))
                     np.testing.assert_array_equal(got[0], eager[i][0])
                     np.testing.assert_array_equal(got[1], eager[i][1])
+                    # the LAK form is written once, before the capture: replays keep it
+                    nf = len(got[0]) // 4
+                    np.testing.assert_array_equal(got[2], 4 * np.arange(nf + 1, dtype=np.int32))
+                    np.testing.assert_array_equal(got[3], np.tile(np.arange(1, 5, dtype=np.int32), nf))
             # new positions, same topology: the replay recomputes from the inputs
             for d, h in zip(dev, hosts):
                 moved = (h.verts.copy())
