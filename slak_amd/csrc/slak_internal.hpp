@@ -56,8 +56,8 @@ void launch_transpose(bool quad, const int32_t* ids, const int32_t* fptr, int32_
 void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
 void launch_number(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
 void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st);
-void launch_refine(bool quad, int scheme, const RefineView& R, hipStream_t st);
-void launch_vertex(bool quad, int scheme, const RefineView& R, hipStream_t st);
+// refine (per corner) and vertex (per vertex) items of a level in one launch
+void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st);
 // LAK form (ptr, vals) of every segment of a level with dst_ptr set; nnz =
 // the level's union corners (= refined faces)
 void launch_lak_form(const SegTable& seg, int64_t nnz, hipStream_t st);

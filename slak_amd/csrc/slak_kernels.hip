@@ -561,8 +561,7 @@ __device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K,
 // memory; every load is issued before the first store.
 
 template <bool QUAD, int SCHEME>
-__global__ void __launch_bounds__(kBlock) k_refine_corner(RefineView R) {
-  const int64_t t64 = block_index() * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void refine_item(const RefineView& R, int64_t t64) {
   if (t64 >= R.nnz) return;
   CornerIn c;
   c.t = (int32_t)t64;
@@ -574,14 +573,14 @@ __global__ void __launch_bounds__(kBlock) k_refine_corner(RefineView R) {
   c.u = R.ids[c.tp];
   corner_topology<QUAD>(R, K, c, R.ci[c.t], R.ci[c.tp]);
   const bool vlo = c.v < c.w;
-  const float4* __restrict__ fp = K.S->dst_verts + K.snv - K.fb;
+  const float4* __restrict__ fp = K.S->dst_verts + K.snv;  // face points, by mesh-local face
   if (c.own) {
     const float4 Pv = R.P[c.v], Pw = R.P[c.w];
     c.plo = vlo ? Pv : Pw;
     c.phi = vlo ? Pw : Pv;
     if (c.s1 >= 0) {
-      c.fa = fp[vlo ? c.f : c.g1];
-      c.fz = fp[vlo ? c.g1 : c.f];
+      c.fa = fp[(vlo ? c.f : c.g1) - K.fb];
+      c.fz = fp[(vlo ? c.g1 : c.f) - K.fb];
     }
   }
   corner_ranks<SCHEME>(R, K, c);
@@ -598,8 +597,7 @@ __global__ void __launch_bounds__(kBlock) k_refine_corner(RefineView R) {
 constexpr int kVertexRegs = 4;
 
 template <bool QUAD, int SCHEME, bool NEXT, bool BND>
-__global__ void __launch_bounds__(kBlock) k_vertex(RefineView R) {
-  const int32_t v = (int32_t)(block_index() * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void vertex_item(const RefineView& R, int32_t v) {
   if (v >= R.nv) return;
   const SegDesc* __restrict__ S = R.seg.d + seg_of(R.seg.vbase, R.seg.n, v);
   const int32_t vb = (int32_t)S->vb, fb = (int32_t)S->fb, cb = (int32_t)S->cb;
@@ -692,6 +690,24 @@ __global__ void __launch_bounds__(kBlock) k_vertex(RefineView R) {
     nrow[i] = make_int2(ncb + 4 * (t - cb), nep + clampi(ci[prevc(t)].x, emax));
     if (nbnd) nbnd[neb + pos + i] = ci[t].y < 0 ? 1 : 0;
   }
+}
+
+// One launch per level: the refine items (one per corner) and the vertex
+// items (one per vertex) are independent once the face points exist, and they
+// write disjoint outputs. Their blocks are interleaved in proportion over the
+// grid, so write-bound refine waves and gather-bound vertex waves share the
+// machine. Block b is a vertex block when floor((b+1) nbv / T) > floor(b nbv / T)
+// (T = nbr + nbv): vertex block floor(b nbv / T), else refine block
+// b - floor(b nbv / T); a bijection onto both block ranges.
+template <bool QUAD, int SCHEME, bool NEXT, bool BND>
+__global__ void __launch_bounds__(kBlock) k_level(RefineView R, int32_t nbr, int32_t nbv) {
+  const int64_t b = block_index();
+  const int64_t T = (int64_t)nbr + nbv;
+  const int64_t before = b * nbv / T, after = (b + 1) * nbv / T;
+  if (after > before)
+    vertex_item<QUAD, SCHEME, NEXT, BND>(R, (int32_t)(before * blockDim.x + threadIdx.x));
+  else
+    refine_item<QUAD, SCHEME>(R, (b - before) * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -985,39 +1001,29 @@ void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st) {
     k_spgemm_fde<false><<<grid_for(L.nv), kBlock, 0, st>>>(L);
 }
 
-void launch_refine(bool quad, int scheme, const RefineView& R, hipStream_t st) {
-  if (R.nnz <= 0) return;
-  const unsigned g = grid_for(R.nnz);
-  if (quad) {
-    if (scheme == kLAK) k_refine_corner<true, kLAK><<<g, kBlock, 0, st>>>(R);
-    else k_refine_corner<true, kSLAK><<<g, kBlock, 0, st>>>(R);
-  } else {
-    if (scheme == kLAK) k_refine_corner<false, kLAK><<<g, kBlock, 0, st>>>(R);
-    else k_refine_corner<false, kSLAK><<<g, kBlock, 0, st>>>(R);
-  }
-}
-
 template <bool QUAD, int SCHEME>
-void launch_vertex_t(const RefineView& R, unsigned g, hipStream_t st) {
+void launch_level_t(const RefineView& R, unsigned g, int32_t nbr, int32_t nbv, hipStream_t st) {
   const bool next = R.ntptr != nullptr, bnd = R.blk != nullptr;
   if (next) {
-    if (bnd) k_vertex<QUAD, SCHEME, true, true><<<g, kBlock, 0, st>>>(R);
-    else k_vertex<QUAD, SCHEME, true, false><<<g, kBlock, 0, st>>>(R);
+    if (bnd) k_level<QUAD, SCHEME, true, true><<<g, kBlock, 0, st>>>(R, nbr, nbv);
+    else k_level<QUAD, SCHEME, true, false><<<g, kBlock, 0, st>>>(R, nbr, nbv);
   } else {
-    if (bnd) k_vertex<QUAD, SCHEME, false, true><<<g, kBlock, 0, st>>>(R);
-    else k_vertex<QUAD, SCHEME, false, false><<<g, kBlock, 0, st>>>(R);
+    if (bnd) k_level<QUAD, SCHEME, false, true><<<g, kBlock, 0, st>>>(R, nbr, nbv);
+    else k_level<QUAD, SCHEME, false, false><<<g, kBlock, 0, st>>>(R, nbr, nbv);
   }
 }
 
-void launch_vertex(bool quad, int scheme, const RefineView& R, hipStream_t st) {
-  if (R.nv <= 0) return;
-  unsigned g = grid_for(R.nv);
+void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st) {
+  const int32_t nbr = R.nnz > 0 ? (int32_t)grid_for(R.nnz) : 0;
+  const int32_t nbv = R.nv > 0 ? (int32_t)grid_for(R.nv) : 0;
+  if (nbr + nbv == 0) return;
+  const unsigned g = (unsigned)(nbr + nbv);
   if (quad) {
-    if (scheme == kLAK) launch_vertex_t<true, kLAK>(R, g, st);
-    else launch_vertex_t<true, kSLAK>(R, g, st);
+    if (scheme == kLAK) launch_level_t<true, kLAK>(R, g, nbr, nbv, st);
+    else launch_level_t<true, kSLAK>(R, g, nbr, nbv, st);
   } else {
-    if (scheme == kLAK) launch_vertex_t<false, kLAK>(R, g, st);
-    else launch_vertex_t<false, kSLAK>(R, g, st);
+    if (scheme == kLAK) launch_level_t<false, kLAK>(R, g, nbr, nbv, st);
+    else launch_level_t<false, kSLAK>(R, g, nbr, nbv, st);
   }
 }
 

@@ -648,22 +648,19 @@ void Plan::record_level(int l) {
   R.next_nnz = (int32_t)L.next_nnz;
   const bool next = L.ntptr != nullptr;
   {
-    // reads ids, (edge, twin) pairs, positions and face points; writes refined
-    // quads, edge points and the next level's (edge, twin) pairs, face- and
-    // edge-point rows of M^T
-    ProfScope ps(st_, L.quad ? "refine_quad" : "refine_poly",
-                 4.0 * L.nnz + 8.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 16.0 * L.nnz + 16.0 * ne +
-                     (L.blk ? 4.0 * ne : 0.0) + poly_b +
-                     (next ? 32.0 * L.nnz + 24.0 * L.nnz + 4.0 * (L.nf + ne) + (L.nbnd ? 16.0 * L.nnz : 0.0) : 0.0));
-    launch_refine(L.quad, scheme_, R, st_);
-  }
-  {
-    // reads M^T rows, positions and face points; writes vertex points and the
-    // next level's vertex rows of M^T
-    ProfScope ps(st_, L.quad ? "vertexpoints_quad" : "vertexpoints_poly",
-                 4.0 * L.nv + 8.0 * L.nnz + 16.0 * L.nf + 16.0 * L.nv + 16.0 * L.nv + (L.blk ? 8.0 * L.nnz : 0.0) +
-                     poly_b + (next ? 8.0 * L.nnz + 8.0 * L.nnz + 4.0 * L.nv : 0.0));
-    launch_vertex(L.quad, scheme_, R, st_);
+    // algorithmic bytes of the fused launch, each array once: reads ids,
+    // (edge, twin) pairs, positions, face points and the M^T rows; writes
+    // refined quads, edge and vertex points and, for a level that feeds
+    // another, its (edge, twin) pairs, M^T rows and offsets, boundary flags
+    // and (SLAK) M^T positions
+    const double reads = 4.0 * L.nnz + 8.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 4.0 * L.nv + 8.0 * L.nnz +
+                         (L.blk ? 4.0 * ne : 0.0) + poly_b + (next && scheme_ == kSLAK ? 4.0 * L.nnz : 0.0);
+    const double writes = 16.0 * L.nnz + 16.0 * ne + 16.0 * L.nv +
+                          (next ? 32.0 * L.nnz + 32.0 * L.nnz + 4.0 * (L.nv + L.nf + ne) +
+                                      (L.nbnd ? 4.0 * (L.nnz + 2.0 * ne) : 0.0) + (scheme_ == kSLAK ? 16.0 * L.nnz : 0.0)
+                                : 0.0);
+    ProfScope ps(st_, L.quad ? "refine_vertex_quad" : "refine_vertex_poly", reads + writes);
+    launch_level(L.quad, scheme_, R, st_);
   }
 }
 
