@@ -49,7 +49,7 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def time_gpu(mesh, levels, scheme, steps, warmup):
+def time_gpu(mesh, levels, scheme, steps, warmup, profile_steps=3):
     from slak_amd import capi
     from slak_amd.mesh import MeshInfo, _subdivide
 
@@ -72,11 +72,13 @@ def time_gpu(mesh, levels, scheme, steps, warmup):
         t1 = time.perf_counter()
         faces = int(st.out_faces)
         # per-stage HIP-event profile over a few extra steps
-        capi.profile_enable(True)
-        for _ in range(3):
-            step()
-        prof = capi.profile_report()
-        capi.profile_enable(False)
+        prof = None
+        if profile_steps:
+            capi.profile_enable(True)
+            for _ in range(profile_steps):
+                step()
+            prof = capi.profile_report()
+            capi.profile_enable(False)
         return (t1 - t0), faces, st, prof
     finally:
         c.freeAndReset()
@@ -264,6 +266,99 @@ def serve_poisson(n_requests, qps_fraction, scheme, seed=5):
     }
 
 
+# profiler stage -> rocprof kernel name prefix ({s}: scheme template argument)
+STAGE_KERNELS = {
+    "refine_vertex_quad": "k_level<true, {s},",
+    "refine_vertex_poly": "k_level<false, {s},",
+    "facepoints_quad": "k_faces<true, true, false>",
+    "facepoints_poly": "k_faces<false, true, false>",
+}
+
+
+def pmc_child(workload, scheme_name):
+    """The profiled program of hbm_traffic(): the 1 GiB copy probe (the
+    counters' calibration), then one warm-up and one timed chain."""
+    import ctypes
+    from slak_amd import capi
+    gbps = ctypes.c_double()
+    capi.check(capi.lib().slak_copy_probe(1 << 30, 1, ctypes.byref(gbps)))
+    scheme = capi.SLAK_SCHEME_LAK if scheme_name == "lak" else capi.SLAK_SCHEME_SLAK
+    time_gpu(make_mesh(workload), WORKLOADS[workload][1], scheme, 1, 1, profile_steps=0)
+
+
+def hbm_traffic(workload, scheme_name):
+    """HBM bytes per dispatch of every kernel of the chain, from two
+    rocprofv3 --pmc runs of pmc_child (FETCH_SIZE, then WRITE_SIZE), each
+    scaled by what the counter reads for the copy probe (1 GiB read, 1 GiB
+    written). Runs before this process touches the GPU; None when rocprofv3
+    is unavailable or a run fails."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    from collections import defaultdict
+
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None
+    out = tempfile.mkdtemp(prefix="slak_pmc_")
+    counts = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", counter.lower(), "--",
+               sys.executable, os.path.abspath(__file__), "--pmc-child", "--workload", workload,
+               "--scheme", scheme_name]
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=180, env=dict(os.environ))
+        except subprocess.TimeoutExpired:
+            return None
+        if r.returncode != 0:
+            return None
+        per = defaultdict(list)
+        for f in glob.glob(os.path.join(out, "**", f"{counter.lower()}_counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    if row.get("Counter_Name") == counter:
+                        name = row["Kernel_Name"]
+                        for pre in ("void ", "slak::(anonymous namespace)::", "slak::"):
+                            name = name.replace(pre, "")
+                        per[name.split("(")[0]].append(float(row["Counter_Value"]) * 1024.0)
+        counts[counter] = per
+    shutil.rmtree(out, ignore_errors=True)
+    fe, wr = counts["FETCH_SIZE"], counts["WRITE_SIZE"]
+    if not fe.get("k_copy_probe") or not wr.get("k_copy_probe"):
+        return None
+    copy = float(1 << 30)
+    cal_f = copy / (sum(fe["k_copy_probe"]) / len(fe["k_copy_probe"]))
+    cal_w = copy / (sum(wr["k_copy_probe"]) / len(wr["k_copy_probe"]))
+    kernels = {}
+    for name in set(fe) | set(wr):
+        f, w = fe.get(name, []), wr.get(name, [])
+        kernels[name] = {"dispatches": max(len(f), len(w)),
+                         "read": cal_f * sum(f) / max(len(f), 1), "write": cal_w * sum(w) / max(len(w), 1)}
+    return {"calibration": {"fetch": round(cal_f, 4), "write": round(cal_w, 4)}, "kernels": kernels}
+
+
+def add_traffic(roof, traffic, scheme_name):
+    """roofline.traffic: average calibrated HBM bytes per dispatch of the
+    dominant stage's kernels (over the same levels as bytes_per_launch)."""
+    if not traffic or roof["kernel"] not in STAGE_KERNELS:
+        return roof
+    prefix = STAGE_KERNELS[roof["kernel"]].format(s=0 if scheme_name == "lak" else 1)
+    sel = [v for k, v in traffic["kernels"].items() if k.startswith(prefix)]
+    n = sum(v["dispatches"] for v in sel)
+    if not n:
+        return roof
+    rd = sum(v["read"] * v["dispatches"] for v in sel) / n
+    wt = sum(v["write"] * v["dispatches"] for v in sel) / n
+    roof["traffic"] = int(rd + wt)
+    roof["traffic_read"] = int(rd)
+    roof["traffic_write"] = int(wt)
+    roof["traffic_calibration"] = traffic["calibration"]
+    roof["traffic_source"] = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the 1 GiB copy probe"
+    return roof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,11 +373,19 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-qps-fraction", type=float, default=0.5)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter runs")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args.workload, args.scheme)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (meshes are independent; "
               "see DESIGN.md for how the work would shard). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    # HBM counters first, in child processes, before this one touches the GPU
+    traffic = None if args.no_traffic else hbm_traffic(args.workload, args.scheme)
 
     from slak_amd import capi
 
@@ -320,7 +423,7 @@ def main():
             "mesh_digest": mesh.digest(),
             "parallelism": "single GPU",
         },
-        "roofline": roofline(prof),
+        "roofline": add_traffic(roofline(prof), traffic, args.scheme),
         "copy_kernel_gbps": round(copy_gbps.value, 1),
         "other_scheme_faces_per_s": round(faces * max(3, args.steps // 2) / secs_o, 1),
         "stages": prof["stages"],
