@@ -103,6 +103,7 @@ struct LevelView {
   int2* trow;            // sorted rows with in-neighbours (written by the symbolic pass), or null
   int32_t* mpos;         // M^T storage position of every corner (SLAK, symbolic pass), or null
   int32_t* ecnt;
+  int32_t* hist_rows;    // zeroed 17-bin histogram of row lengths (symbolic pass), or null
   const int32_t* eptr;
   int32_t* ci;           // (edge id, twin) pairs per corner, written by the number pass
   int32_t* bnd;          // union edge boundary flags (number pass), or null: closed level

@@ -38,15 +38,14 @@ struct Error : std::runtime_error {
 // reads SLAK_XCD_REMAP once and sets the device-side option (call before capture)
 void init_device_options();
 void launch_fill_i32(int32_t* p, int64_t n, int32_t value, hipStream_t st);
-// also zeroes zero[0..nzero) (the counters of the analysis that follows)
+// also zeroes zero[0..nzero) (the counters of the analysis that follows) and,
+// when hist_faces is set, adds the face orders to that zeroed 17-bin histogram
 void launch_gather(const SegTable& seg, int32_t nf_total, bool use_vals, int32_t* ids_pos, int32_t* fptr,
-                   int32_t* cface, int* flags, int32_t* zero, int32_t nzero, hipStream_t st);
+                   int32_t* cface, int* flags, int32_t* zero, int32_t nzero, int32_t* hist_faces, hipStream_t st);
 void launch_gather_verts(const SegTable& seg, int32_t nv_total, float4* P, hipStream_t st);
 constexpr int kRowHistBins = 17;  // row lengths 0..15, 16 = longer
-// hist must be zeroed; also writes bounds[k] = eptr[vbase[k]], k = 0..nseg
-void launch_row_hist(const int32_t* tptr, int32_t nv, const int32_t* fptr, int32_t nf, int32_t face_order,
-                     int32_t* hist, const int64_t* vbase, int32_t nseg, const int32_t* eptr, int32_t* bounds,
-                     hipStream_t st);
+// bounds[k] = eptr[vbase[k]], k = 0..nseg
+void launch_seg_bounds(const int64_t* vbase, int32_t nseg, const int32_t* eptr, int32_t* bounds, hipStream_t st);
 void launch_copy_probe(const float4* a, float4* b, int64_t n, hipStream_t st);
 // guard (or null): the kernel does nothing when it holds an input-validation flag
 void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int32_t* fptr, const float4* P, int32_t nf,
@@ -54,7 +53,8 @@ void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int3
 void launch_transpose(bool quad, const int32_t* ids, const int32_t* fptr, int32_t nf, const int32_t* tptr,
                       int32_t* cnt, int32_t* tids, const int* guard, hipStream_t st);
 void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
-void launch_number(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
+// first level: edge numbering and face points in one launch
+void launch_number_faces(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
 void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st);
 // refine (per corner) and vertex (per vertex) items of a level in one launch
 void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st);
@@ -62,7 +62,9 @@ void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st);
 // the level's union corners (= refined faces)
 void launch_lak_form(const SegTable& seg, int64_t nnz, hipStream_t st);
 int64_t scan_partials_len(int64_t n);
-void launch_scan(const int32_t* in, int32_t* out, int64_t n, int64_t* partial, int* flags, hipStream_t st);
+// out[i] = sum(in[0..i)), out[n] = total (also to *total_out when given)
+void launch_scan(const int32_t* in, int32_t* out, int64_t n, int64_t* partial, int* flags, hipStream_t st,
+                 int32_t* total_out = nullptr);
 
 // ---------------------------------------------------------------------------
 // caching device allocator (one per device)

@@ -82,12 +82,44 @@ __global__ void __launch_bounds__(kBlock) k_copy_probe(const float4* __restrict_
 // first level: validate each caller mesh and gather it into the union in
 // position order (sortKeyValuePairsSegmentedInPlace, LAKCatmullClarkKernels.cuh:147-180)
 
-__global__ void k_gather(SegTable seg, int32_t nf_total, int use_vals, int32_t* __restrict__ ids_pos,
+constexpr int kHistBins = 17;  // row lengths 0..15, 16 = longer
+
+// Block-local histogram in LDS, added to a zeroed global one (one atomic per bin).
+struct BlockHist {
+  int32_t* lds;
+  __device__ __forceinline__ void init() {
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) lds[i] = 0;
+    __syncthreads();
+  }
+  __device__ __forceinline__ void add(int n) { atomicAdd(&lds[n < kHistBins - 1 ? n : kHistBins - 1], 1); }
+  __device__ __forceinline__ void flush(int32_t* global) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
+      if (lds[i]) atomicAdd(&global[i], lds[i]);
+  }
+};
+
+__device__ void gather_face(const SegTable& seg, int32_t f, int32_t nf_total, int use_vals, int32_t* __restrict__ ids_pos,
+                            int32_t* __restrict__ fptr, int32_t* __restrict__ cface, int* __restrict__ flags,
+                            BlockHist* hist);
+
+__global__ void __launch_bounds__(kBlock) k_gather(SegTable seg, int32_t nf_total, int use_vals, int32_t* __restrict__ ids_pos,
                          int32_t* __restrict__ fptr, int32_t* __restrict__ cface, int* __restrict__ flags,
-                         int32_t* __restrict__ zero, int32_t nzero) {
-  int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
+                         int32_t* __restrict__ zero, int32_t nzero, int32_t* __restrict__ hist_faces) {
+  __shared__ int32_t h[kHistBins];
+  BlockHist bh{h};
+  if (hist_faces) bh.init();
+  const int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (f < nzero) zero[f] = 0;  // counters of the analysis that follows
-  if (f >= nf_total) return;
+  if (f < nf_total) gather_face(seg, f, nf_total, use_vals, ids_pos, fptr, cface, flags, hist_faces ? &bh : nullptr);
+  if (hist_faces) bh.flush(hist_faces);
+}
+
+// first level: validate caller face f and gather it into the union in
+// position order; face order into the block histogram
+__device__ void gather_face(const SegTable& seg, int32_t f, int32_t nf_total, int use_vals, int32_t* __restrict__ ids_pos,
+                            int32_t* __restrict__ fptr, int32_t* __restrict__ cface, int* __restrict__ flags,
+                            BlockHist* hist) {
   const int k = seg_of(seg.fbase, seg.n, f);
   const SegDesc* S = seg.d + k;
   const int32_t lf = (int32_t)(f - S->fb);
@@ -107,6 +139,7 @@ __global__ void k_gather(SegTable seg, int32_t nf_total, int use_vals, int32_t* 
     return;
   }
   const int32_t order = (int32_t)(e - s);
+  if (hist) hist->add(order);
   if (order < 3) bad |= kFlagBadOrder;
   if (S->require_quad && (order != 4 || s != 4ll * lf)) bad |= kFlagBadQuad;
   const int64_t cb = S->cb;
@@ -152,44 +185,21 @@ __global__ void k_gather_verts(SegTable seg, int32_t nv_total, float4* __restric
   P[v] = S->src_verts[v - S->vb];
 }
 
-// Row-length histograms of a level: M^T rows (valence, bins 0..15, 16 = more)
-// and face orders (same bins), block-local in LDS then one atomic per bin
-// into a zeroed hist. Block 0 also writes eptr at every segment's first
-// vertex: bounds[k] = eptr[vbase[k]], k = 0..n.
-constexpr int kHistBins = 17;
-
-__global__ void __launch_bounds__(kBlock) k_row_hist(const int32_t* __restrict__ tptr, int32_t nv,
-                                                      const int32_t* __restrict__ fptr, int32_t nf, int32_t face_order,
-                                                      int32_t* __restrict__ hist, const int64_t* __restrict__ vbase,
-                                                      int32_t nseg, const int32_t* __restrict__ eptr,
-                                                      int32_t* __restrict__ bounds) {
-  __shared__ int32_t h[2 * kHistBins];
-  for (int i = threadIdx.x; i < 2 * kHistBins; i += blockDim.x) h[i] = 0;
-  if (blockIdx.x == 0)
-    for (int k = threadIdx.x; k <= nseg; k += blockDim.x) bounds[k] = eptr[vbase[k]];
-  __syncthreads();
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t v = i; v < nv; v += stride) {
-    int n = tptr[v + 1] - tptr[v];
-    atomicAdd(&h[n < kHistBins - 1 ? n : kHistBins - 1], 1);
-  }
-  for (int64_t f = i; f < nf; f += stride) {
-    int n = fptr ? fptr[f + 1] - fptr[f] : face_order;
-    atomicAdd(&h[kHistBins + (n < kHistBins - 1 ? n : kHistBins - 1)], 1);
-  }
-  __syncthreads();
-  for (int b = threadIdx.x; b < 2 * kHistBins; b += blockDim.x)
-    if (h[b]) atomicAdd(&hist[b], h[b]);
+// eptr at every segment's first vertex: bounds[k] = eptr[vbase[k]], k = 0..n
+// (multi-mesh plans; a single mesh takes its total from the scan)
+__global__ void k_seg_bounds(const int64_t* __restrict__ vbase, int32_t n, const int32_t* __restrict__ eptr,
+                             int32_t* __restrict__ bounds) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k <= n) bounds[k] = eptr[vbase[k]];
 }
 
 // ---------------------------------------------------------------------------
 // valence count (+ face points into each segment's destination)
 
 template <bool QUAD, bool FP, bool COUNT>
-__global__ void k_faces(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr, const float4* __restrict__ P,
-                        int32_t nf, SegTable seg, int32_t* __restrict__ cnt, const int* __restrict__ guard) {
-  int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void faces_item(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr,
+                                           const float4* __restrict__ P, int32_t nf, const SegTable& seg,
+                                           int32_t* __restrict__ cnt, const int* __restrict__ guard, int32_t f) {
   if (f >= nf) return;
   if (guard && (*guard & kFlagsInput)) return;
   float4* dst = nullptr;
@@ -216,6 +226,12 @@ __global__ void k_faces(const int32_t* __restrict__ ids, const int32_t* __restri
     }
     if (FP) *dst = f4div(acc, (float)(e - s));
   }
+}
+
+template <bool QUAD, bool FP, bool COUNT>
+__global__ void k_faces(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr, const float4* __restrict__ P,
+                        int32_t nf, SegTable seg, int32_t* __restrict__ cnt, const int* __restrict__ guard) {
+  faces_item<QUAD, FP, COUNT>(ids, fptr, P, nf, seg, cnt, guard, (int32_t)(block_index() * blockDim.x + threadIdx.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -259,12 +275,25 @@ __device__ void sort_row_in_place(int32_t* a, int n) {
 }
 
 template <bool QUAD, int SCHEME, int CAP>
+__device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, BlockHist* hist);
+
+template <bool QUAD, int SCHEME, int CAP>
 __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
-  int32_t v = (int32_t)(block_index() * blockDim.x + threadIdx.x);
-  if (v >= L.nv) return;
-  if (*L.flags & kFlagsInput) return;
+  __shared__ int32_t h[kHistBins];
+  BlockHist bh{h};
+  if (L.hist_rows) bh.init();
+  const int32_t v = (int32_t)(block_index() * blockDim.x + threadIdx.x);
+  if (v < L.nv && !(*L.flags & kFlagsInput)) symbolic_row<QUAD, SCHEME, CAP>(L, v, L.hist_rows ? &bh : nullptr);
+  if (L.hist_rows) bh.flush(L.hist_rows);
+}
+
+// mapped-SpGEMM row sizing of vertex v (edges it owns), validation, the
+// sorted row with in-neighbours; row length into the block histogram
+template <bool QUAD, int SCHEME, int CAP>
+__device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, BlockHist* hist) {
   int32_t s = L.tptr[v];
   int n = L.tptr[v + 1] - s;
+  if (hist) hist->add(n);
   int32_t count = 0, bcount = 0;
   int bad = 0;
   int2* __restrict__ trow = L.trow;
@@ -298,8 +327,26 @@ __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
 }
 
 template <bool QUAD, int SCHEME, int CAP>
-__global__ void __launch_bounds__(kBlock) k_number(LevelView L) {
-  int32_t v = (int32_t)(block_index() * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void number_row(const LevelView& L, int32_t v);
+template <bool QUAD, bool FP, bool COUNT>
+__device__ __forceinline__ void faces_item(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr,
+                                           const float4* __restrict__ P, int32_t nf, const SegTable& seg,
+                                           int32_t* __restrict__ cnt, const int* __restrict__ guard, int32_t f);
+
+// First level: the edge numbering (blocks [0, nbn)) and the face points (the
+// other blocks) are independent; one launch runs both.
+template <bool QUAD, int SCHEME, int CAP>
+__global__ void __launch_bounds__(kBlock) k_number_faces(LevelView L, int32_t nbn) {
+  const int64_t b = block_index();
+  if (b < nbn)
+    number_row<QUAD, SCHEME, CAP>(L, (int32_t)(b * blockDim.x + threadIdx.x));
+  else
+    faces_item<QUAD, true, false>(L.ids, L.fptr, L.P, L.nf, L.seg, nullptr, nullptr,
+                                  (int32_t)((b - nbn) * blockDim.x + threadIdx.x));
+}
+
+template <bool QUAD, int SCHEME, int CAP>
+__device__ __forceinline__ void number_row(const LevelView& L, int32_t v) {
   if (v >= L.nv) return;
   int32_t s = L.tptr[v];
   int n = L.tptr[v + 1] - s;
@@ -768,7 +815,8 @@ constexpr int64_t kScanSmall = 4 * (int64_t)kScanSmallChunk;
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
 
 __global__ void __launch_bounds__(kScanSmallThreads) k_scan_small(const int32_t* __restrict__ in, int64_t n,
-                                                                  int32_t* __restrict__ out, int* __restrict__ flags) {
+                                                                  int32_t* __restrict__ out, int* __restrict__ flags,
+                                                                  int32_t* __restrict__ total_out) {
   __shared__ int32_t buf[kScanSmallChunk + kScanSmallChunk / 32];
   __shared__ int64_t wsum[kScanSmallThreads / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -831,6 +879,7 @@ __global__ void __launch_bounds__(kScanSmallThreads) k_scan_small(const int32_t*
   if (tid == 0) {
     if (carry > 0x7fffffffll) atomicOr(flags, kFlagOverflow);
     out[n] = (int32_t)carry;
+    if (total_out) *total_out = (int32_t)carry;
   }
 }
 
@@ -848,7 +897,8 @@ __global__ void __launch_bounds__(kBlock) k_scan_reduce(const int32_t* __restric
 }
 
 __global__ void __launch_bounds__(kBlock) k_scan_partials(int64_t* __restrict__ partial, int64_t ntiles,
-                                                          int32_t* __restrict__ out, int64_t n, int* __restrict__ flags) {
+                                                          int32_t* __restrict__ out, int64_t n, int* __restrict__ flags,
+                                                          int32_t* __restrict__ total_out) {
   __shared__ int64_t lds[kBlock / 64];
   int64_t carry = 0;
   for (int64_t b = 0; b < ntiles; b += kBlock) {
@@ -862,6 +912,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_partials(int64_t* __restrict__ 
   if (threadIdx.x == 0) {
     if (carry > 0x7fffffffll) atomicOr(flags, kFlagOverflow);
     out[n] = (int32_t)carry;
+    if (total_out) *total_out = (int32_t)carry;
   }
 }
 
@@ -907,11 +958,11 @@ void launch_fill_i32(int32_t* p, int64_t n, int32_t value, hipStream_t st) {
 }
 
 void launch_gather(const SegTable& seg, int32_t nf_total, bool use_vals, int32_t* ids_pos, int32_t* fptr,
-                   int32_t* cface, int* flags, int32_t* zero, int32_t nzero, hipStream_t st) {
+                   int32_t* cface, int* flags, int32_t* zero, int32_t nzero, int32_t* hist_faces, hipStream_t st) {
   const int32_t n = std::max(nf_total, nzero);
   if (n <= 0) return;
   k_gather<<<grid_for(n), kBlock, 0, st>>>(seg, nf_total, use_vals ? 1 : 0, ids_pos, fptr, cface, flags, zero,
-                                          nzero);
+                                          nzero, hist_faces);
 }
 
 void launch_gather_verts(const SegTable& seg, int32_t nv_total, float4* P, hipStream_t st) {
@@ -919,13 +970,8 @@ void launch_gather_verts(const SegTable& seg, int32_t nv_total, float4* P, hipSt
   k_gather_verts<<<grid_for(nv_total), kBlock, 0, st>>>(seg, nv_total, P);
 }
 
-void launch_row_hist(const int32_t* tptr, int32_t nv, const int32_t* fptr, int32_t nf, int32_t face_order,
-                     int32_t* hist, const int64_t* vbase, int32_t nseg, const int32_t* eptr, int32_t* bounds,
-                     hipStream_t st) {
-  int64_t g = (std::max<int64_t>(nv, nf) + kBlock - 1) / kBlock;
-  if (g > 1024) g = 1024;
-  if (g < 1) g = 1;
-  k_row_hist<<<(unsigned)g, kBlock, 0, st>>>(tptr, nv, fptr, nf, face_order, hist, vbase, nseg, eptr, bounds);
+void launch_seg_bounds(const int64_t* vbase, int32_t n, const int32_t* eptr, int32_t* bounds, hipStream_t st) {
+  k_seg_bounds<<<grid_for(n + 1), kBlock, 0, st>>>(vbase, n, eptr, bounds);
 }
 
 void launch_copy_probe(const float4* a, float4* b, int64_t n, hipStream_t st) {
@@ -987,10 +1033,28 @@ void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStre
   SLAK_ROW_DISPATCH(k_symbolic, quad, scheme, cap, g, L, st);
 }
 
-void launch_number(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st) {
-  if (L.nv <= 0) return;
-  unsigned g = grid_for(L.nv);
-  SLAK_ROW_DISPATCH(k_number, quad, scheme, cap, g, L, st);
+void launch_number_faces(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st) {
+  const int32_t nbn = L.nv > 0 ? (int32_t)grid_for(L.nv) : 0;
+  const int32_t nbf = L.nf > 0 ? (int32_t)grid_for(L.nf) : 0;
+  if (nbn + nbf == 0) return;
+  const unsigned g = (unsigned)(nbn + nbf);
+  if (quad) {
+    if (scheme == kLAK) {
+      if (cap <= 4) k_number_faces<true, kLAK, 4><<<g, kBlock, 0, st>>>(L, nbn);
+      else k_number_faces<true, kLAK, 8><<<g, kBlock, 0, st>>>(L, nbn);
+    } else {
+      if (cap <= 4) k_number_faces<true, kSLAK, 4><<<g, kBlock, 0, st>>>(L, nbn);
+      else k_number_faces<true, kSLAK, 8><<<g, kBlock, 0, st>>>(L, nbn);
+    }
+  } else {
+    if (scheme == kLAK) {
+      if (cap <= 4) k_number_faces<false, kLAK, 4><<<g, kBlock, 0, st>>>(L, nbn);
+      else k_number_faces<false, kLAK, 8><<<g, kBlock, 0, st>>>(L, nbn);
+    } else {
+      if (cap <= 4) k_number_faces<false, kSLAK, 4><<<g, kBlock, 0, st>>>(L, nbn);
+      else k_number_faces<false, kSLAK, 8><<<g, kBlock, 0, st>>>(L, nbn);
+    }
+  }
 }
 
 void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st) {
@@ -1034,14 +1098,15 @@ void launch_lak_form(const SegTable& seg, int64_t nnz, hipStream_t st) {
 
 int64_t scan_partials_len(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
 
-void launch_scan(const int32_t* in, int32_t* out, int64_t n, int64_t* partial, int* flags, hipStream_t st) {
+void launch_scan(const int32_t* in, int32_t* out, int64_t n, int64_t* partial, int* flags, hipStream_t st,
+                 int32_t* total_out) {
   if (n <= kScanSmall) {
-    k_scan_small<<<1, kScanSmallThreads, 0, st>>>(in, n, out, flags);
+    k_scan_small<<<1, kScanSmallThreads, 0, st>>>(in, n, out, flags, total_out);
     return;
   }
   int64_t ntiles = (n + kScanTile - 1) / kScanTile;
   k_scan_reduce<<<(unsigned)ntiles, kBlock, 0, st>>>(in, n, partial);
-  k_scan_partials<<<1, kBlock, 0, st>>>(partial, ntiles, out, n, flags);
+  k_scan_partials<<<1, kBlock, 0, st>>>(partial, ntiles, out, n, flags, total_out);
   k_scan_down<<<(unsigned)ntiles, kBlock, 0, st>>>(in, n, partial, out);
 }
 

@@ -96,11 +96,13 @@ Plan::~Plan() {
   if (!arena_.empty()) (void)hipStreamSynchronize(st_);
 }
 
+// Reads the flags word and the count ints after it in the state block in one
+// copy (count = 0: flags only) and waits for it.
 int Plan::readback(const int32_t* src, int count, int64_t* dst) {
+  (void)src;
   int64_t* pin = pinned_buffer((size_t)count / 2 + 2);
   int32_t* p32 = reinterpret_cast<int32_t*>(pin);
-  SLAK_HIP_CHECK(hipMemcpyAsync(p32, d_flags_, sizeof(int), hipMemcpyDeviceToHost, st_));
-  if (count > 0) SLAK_HIP_CHECK(hipMemcpyAsync(p32 + 1, src, count * sizeof(int32_t), hipMemcpyDeviceToHost, st_));
+  SLAK_HIP_CHECK(hipMemcpyAsync(p32, d_flags_, (1 + count) * sizeof(int32_t), hipMemcpyDeviceToHost, st_));
   SLAK_HIP_CHECK(hipStreamSynchronize(st_));
   ++syncs_;
   for (int i = 0; i < count; ++i) dst[i] = p32[1 + i];
@@ -110,7 +112,7 @@ int Plan::readback(const int32_t* src, int count, int64_t* dst) {
 // Uploads the segment tables (SegDesc, vertex, face and corner bases) of levels
 // [l0, l1) in one copy from a plan-owned staging buffer (one per call site,
 // so a pending copy never sees its source rewritten).
-void Plan::upload_tables(int l0, int l1, std::vector<char>& staging) {
+void Plan::upload_tables(int l0, int l1, std::vector<char>& staging, int64_t state_ints) {
   auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
   size_t total = 0;
   std::vector<size_t> offs;
@@ -119,9 +121,15 @@ void Plan::upload_tables(int l0, int l1, std::vector<char>& staging) {
     offs.push_back(total);
     total = align(total + n * sizeof(SegDesc)) + align(3 * (n + 1) * sizeof(int64_t));
   }
+  const size_t state_off = total;
+  total += align((size_t)state_ints * sizeof(int32_t));
   if (total == 0) return;
   staging.assign(total, 0);
   char* d = arena_.get_n<char>((int64_t)total);
+  if (state_ints > 0) {  // the plan's state block, zeros
+    d_state_ = reinterpret_cast<int32_t*>(d + state_off);
+    d_flags_ = d_state_;
+  }
   for (int l = l0; l < l1; ++l) {
     LevelPlan& L = levels_[l];
     const size_t n = L.segs.size();
@@ -135,7 +143,12 @@ void Plan::upload_tables(int l0, int l1, std::vector<char>& staging) {
     L.d_fbase = reinterpret_cast<int64_t*>(d + o) + (n + 1);
     std::memcpy(staging.data() + o + 2 * (n + 1) * sizeof(int64_t), L.cbase.data(), (n + 1) * sizeof(int64_t));
     L.d_cbase = reinterpret_cast<int64_t*>(d + o) + 2 * (n + 1);
-    if (!L.d_mflags) L.d_mflags = arena_.get_n<int>((int64_t)std::max<size_t>(n, 1));
+    if (!L.d_mflags) {
+      if (l == 0 && d_state_)  // the first level's mesh flags live in the state block
+        L.d_mflags = d_state_ + 1 + 2 * kRowHistBins + (n + 1);
+      else
+        L.d_mflags = arena_.get_n<int>((int64_t)std::max<size_t>(n, 1));
+    }
   }
   SLAK_HIP_CHECK(hipMemcpyAsync(d, staging.data(), total, hipMemcpyHostToDevice, st_));
 }
@@ -212,12 +225,8 @@ void Plan::plan_first_level() {
     P1_ = arena_.get_n<float4>(L.nv);
     L.P = P1_;
   }
-  // [row-length histograms | per-mesh edge bounds | valence counters], zeroed
-  // by the gather; the first two are read back together
-  const int64_t nseg = (int64_t)L.segs.size();
-  d_counters_ = arena_.get_n<int32_t>(2 * kRowHistBins + nseg + 1 + L.nv);
-  n_counters_ = 2 * kRowHistBins + nseg + 1 + L.nv;
-  L.cnt = d_counters_ + 2 * kRowHistBins + nseg + 1;
+  d_cnt_ = arena_.get_n<int32_t>(L.nv);
+  L.cnt = d_cnt_;
   L.tptr = arena_.get_n<int32_t>(L.nv + 1);
   L.tids = arena_.get_n<int32_t>(L.nnz);
   L.trow = arena_.get_n<int2>(L.nnz);
@@ -258,6 +267,7 @@ void Plan::record_analysis(LevelPlan& L) {
   V.trow = L.trow;
   V.mpos = L.mpos;
   V.ecnt = L.ecnt;
+  V.hist_rows = d_state_ + 1;
   V.flags = d_flags_;
   V.seg = L.table();
   {
@@ -266,8 +276,10 @@ void Plan::record_analysis(LevelPlan& L) {
     launch_symbolic(L.quad, scheme_, L.cap, V, st_);
   }
   {
+    // a single mesh's edge total goes straight to its read-back slot
+    int32_t* bounds = d_state_ + 1 + 2 * kRowHistBins;
     ProfScope ps(st_, "scan", 8.0 * L.nv + 4);
-    launch_scan(L.ecnt, L.eptr, L.nv, d_partial_, d_flags_, st_);
+    launch_scan(L.ecnt, L.eptr, L.nv, d_partial_, d_flags_, st_, L.segs.size() == 1 ? bounds + 1 : nullptr);
   }
 }
 
@@ -289,29 +301,29 @@ void Plan::prepare() {
     check_i32((int64_t)J.in.nfaces, "nfaces");
     check_i32((int64_t)J.in.nnz, "nnz");
   }
-  d_flags_ = arena_.get_n<int>(1);
   bool any = false;
   for (auto& J : jobs_) any |= J.levels > 0;
   if (!any) {
+    d_flags_ = arena_.get_n<int>(1);
+    launch_fill_i32(d_flags_, 1, 0, st_);
     prepared_ = true;
     return;
   }
   plan_first_level();
   LevelPlan& L1 = levels_[0];
   const int nseg = (int)L1.segs.size();
-  upload_tables(0, 1, staging_first_);
+  n_state_ = 1 + 2 * kRowHistBins + (nseg + 1) + nseg;
+  upload_tables(0, 1, staging_first_, n_state_);
   int64_t need = scan_partials_len(L1.nv);
   d_partial_ = arena_.get_n<int64_t>(need);
   partial_len_ = need;
 
-  // 1. validate + gather
-  launch_fill_i32(d_flags_, 1, 0, st_);
-  launch_fill_i32(L1.d_mflags, nseg, 0, st_);
+  // 1. validate + gather (the state block arrived zeroed with the tables)
   {
     ProfScope ps(st_, "gather", 4.0 * (L1.nf + 1) + 12.0 * L1.nnz + (L1.quad ? 0.0 : 8.0 * L1.nnz + 4.0 * L1.nf));
     launch_gather(L1.table(), (int32_t)L1.nf, scheme_ == kLAK, const_cast<int32_t*>(L1.ids),
-                  const_cast<int32_t*>(L1.fptr), const_cast<int32_t*>(L1.cface), d_flags_, d_counters_,
-                  (int32_t)n_counters_, st_);
+                  const_cast<int32_t*>(L1.fptr), const_cast<int32_t*>(L1.cface), d_flags_, d_cnt_, (int32_t)L1.nv,
+                  d_state_ + 1 + kRowHistBins, st_);
   }
   if (P1_) launch_gather_verts(L1.table(), (int32_t)L1.nv, P1_, st_);
   auto fail_if = [&](int fl) {
@@ -330,14 +342,12 @@ void Plan::prepare() {
   // (its kernels do nothing on an input the gather flagged; the flags come
   // back with the edge counts in one round trip)
   record_analysis(L1);
-  int32_t* d_hist = d_counters_;
-  int32_t* d_bounds = d_counters_ + 2 * kRowHistBins;
-  launch_row_hist(L1.tptr, (int32_t)L1.nv, L1.fptr, (int32_t)L1.nf, L1.quad ? 4 : 0, d_hist, L1.d_vbase, nseg,
-                  L1.eptr, d_bounds, st_);
+  int32_t* d_bounds = d_state_ + 1 + 2 * kRowHistBins;
+  if (nseg > 1) launch_seg_bounds(L1.d_vbase, nseg, L1.eptr, d_bounds, st_);
   std::vector<int64_t> hb(2 * kRowHistBins + nseg + 1, 0);
   bool have_hist = false;
   if (!all_known) {
-    fail_if(readback(d_hist, 2 * kRowHistBins + nseg + 1, hb.data()));
+    fail_if(readback(nullptr, 2 * kRowHistBins + nseg + 1, hb.data()));
     have_hist = true;
   } else {
     if (!all_trusted) fail_if(readback(nullptr, 0, nullptr));
@@ -612,15 +622,17 @@ void Plan::record_level(int l) {
     V.bnd = L.bnd;
     V.flags = d_flags_;
     V.seg = L.table();
-    ProfScope ps(st_, L.quad ? "spgemm_number_quad" : "spgemm_number_poly",
-                 12.0 * L.nv + 12.0 * L.nnz + 8.0 * L.nnz + (L.bnd ? 4.0 * ne : 0.0) + poly_b);
-    launch_number(L.quad, scheme_, L.cap, V, st_);
+    // with the first level's face points in the same launch
+    ProfScope ps(st_, L.quad ? "number_facepoints_quad" : "number_facepoints_poly",
+                 12.0 * L.nv + 12.0 * L.nnz + 8.0 * L.nnz + (L.bnd ? 4.0 * ne : 0.0) + poly_b + 4.0 * L.nnz +
+                     16.0 * L.nv + 16.0 * L.nf);
+    launch_number_faces(L.quad, scheme_, L.cap, V, st_);
   }
   if (L.blk) {
     ProfScope ps(st_, "scan", 8.0 * ne + 4);
     launch_scan(L.bnd, L.blk, L.ne, d_partial_, d_flags_, st_);
   }
-  {
+  if (l > 1) {
     ProfScope ps(st_, L.quad ? "facepoints_quad" : "facepoints_poly",
                  4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + (L.quad ? 0.0 : 4.0 * (L.nf + 1)));
     launch_faces(L.quad, true, false, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), nullptr, nullptr, st_);
@@ -692,13 +704,12 @@ void Plan::launch(bool include_analysis) {
   LevelPlan& L1 = levels_[0];
   if (!lak_forms_in_place_) launch_lak_forms();
   if (include_analysis) {
-    launch_fill_i32(d_flags_, 1, 0, st_);
-    launch_fill_i32(L1.d_mflags, (int64_t)L1.segs.size(), 0, st_);
+    launch_fill_i32(d_state_, n_state_, 0, st_);  // a fill kernel: graph-safe
     {
       ProfScope ps(st_, "gather", 4.0 * (L1.nf + 1) + 12.0 * L1.nnz + (L1.quad ? 0.0 : 8.0 * L1.nnz + 4.0 * L1.nf));
       launch_gather(L1.table(), (int32_t)L1.nf, scheme_ == kLAK, const_cast<int32_t*>(L1.ids),
-                    const_cast<int32_t*>(L1.fptr), const_cast<int32_t*>(L1.cface), d_flags_, d_counters_,
-                    (int32_t)n_counters_, st_);
+                    const_cast<int32_t*>(L1.fptr), const_cast<int32_t*>(L1.cface), d_flags_, d_cnt_,
+                    (int32_t)L1.nv, d_state_ + 1 + kRowHistBins, st_);
     }
     if (P1_) {
       ProfScope ps(st_, "gather_verts", 32.0 * L1.nv);

@@ -116,7 +116,8 @@ class Plan {
   void plan_first_level();
   void record_analysis(LevelPlan& L);
   void record_level(int l);
-  void upload_tables(int l0, int l1, std::vector<char>& staging);
+  // state_ints > 0: also carries the plan's zeroed state block (first upload)
+  void upload_tables(int l0, int l1, std::vector<char>& staging, int64_t state_ints = 0);
   int readback(const int32_t* src, int count, int64_t* dst);
 
   int scheme_;
@@ -128,8 +129,11 @@ class Plan {
   int* d_flags_ = nullptr;
   int64_t* d_partial_ = nullptr;
   int64_t partial_len_ = 0;
-  int32_t* d_counters_ = nullptr;  // first level: histograms | edge bounds | valence counters
-  int64_t n_counters_ = 0;
+  // per-plan device state, zeroed by the first table upload:
+  // [flags | row hist (17) | face hist (17) | edge bounds (nseg + 1) | mesh flags (nseg)]
+  int32_t* d_state_ = nullptr;
+  int64_t n_state_ = 0;
+  int32_t* d_cnt_ = nullptr;  // first level: valence counters (zeroed by the gather)
   std::vector<char> staging_first_, staging_all_;  // segment table uploads
   // first-level union gather (multi-mesh)
   float4* P1_ = nullptr;
