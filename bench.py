@@ -359,6 +359,39 @@ def add_traffic(roof, traffic, scheme_name):
     return roof
 
 
+def io_floor_bytes(mesh, levels, lak_form):
+    """Implementation-independent I/O floor of the chain: per level, read the
+    level's ids (+ ptr + vals in LAK form) and positions, write the refined
+    ids (+ ptr + vals) and positions; summed over the levels. Sizes follow
+    the closed forms (n_f' = nnz, nnz' = 4 nnz, n_e = (nnz - n_ext)/2 + n_ext,
+    n_v' = n_v + n_f + n_e, n_ext' = 2 n_ext)."""
+    import numpy as np
+    ptr = mesh.face_ptr().astype(np.int64)
+    ids = mesh.ids.astype(np.int64)
+    nxt = np.arange(len(ids)) + 1
+    nxt[ptr[1:] - 1] = ptr[:-1]  # cyclic successor within each face
+    a, b = ids, ids[nxt]
+    he = set(zip(a.tolist(), b.tolist()))
+    n_ext = sum(1 for (x, y) in he if (y, x) not in he)
+    nv, nf, nnz = int(mesh.nverts), int(mesh.nfaces), int(len(ids))
+    total = 0.0
+    for _ in range(levels):
+        ne = (nnz - n_ext) // 2 + n_ext
+        nv2, nf2, nnz2 = nv + nf + ne, nnz, 4 * nnz
+        rd = 4 * nnz + 16 * nv + ((4 * (nf + 1) + 4 * nnz) if lak_form else 0)
+        wr = 4 * nnz2 + 16 * nv2 + ((4 * (nf2 + 1) + 4 * nnz2) if lak_form else 0)
+        total += rd + wr
+        nv, nf, nnz, n_ext = nv2, nf2, nnz2, 2 * n_ext
+    return total
+
+
+def io_floor(mesh, levels, lak_form, secs_per_step):
+    b = io_floor_bytes(mesh, levels, lak_form)
+    gbps = b / secs_per_step / 1e9
+    return {"bytes_per_chain": int(b), "gbps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4),
+            "note": "per level: read ids(+ptr+vals)+positions, write refined ids(+ptr+vals)+positions"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -424,6 +457,7 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": add_traffic(roofline(prof), traffic, args.scheme),
+        "io_floor": io_floor(mesh, levels, args.scheme == "lak", secs / args.steps),
         "copy_kernel_gbps": round(copy_gbps.value, 1),
         "other_scheme_faces_per_s": round(faces * max(3, args.steps // 2) / secs_o, 1),
         "stages": prof["stages"],

@@ -121,6 +121,22 @@ int slak_spgemm_mapped(const slak_mesh* cmesh, int32_t** F_ptr, int32_t** F_ids,
                        int32_t** D_ptr, int32_t** D_ids, int32_t** D_vals, int32_t** E_ptr, int32_t** E_ids,
                        int32_t** E_vals, uint64_t* nedges, uint64_t* nextern, void* stream);
 
+/* ---- reduced / unreduced forms of homogeneous (TRI, QUAD) meshes ---- */
+
+/* GPU::CSCMesh::reduce (include/Mesh.h:305-334): each face's (ids, vals) is
+ * reordered in place by vals, so that ids are in corner-position order, and
+ * is_reduced is set. A mesh already reduced is left as it is. POLY meshes are
+ * rejected ("mesh type not homogeneous"), as is an unreduced mesh without
+ * vals. Synchronous; device pointers in *mesh. */
+int slak_mesh_reduce(slak_mesh* mesh, void* stream);
+/* GPU::CSCMesh::unreduce (include/Mesh.h:336-377): a missing ptr is allocated
+ * with stride offsets and missing vals with 1-based corner positions (both
+ * freed with slak_free); each face's (ids, vals) is then reordered in place by
+ * ids and is_reduced is cleared. The reference fills missing vals with
+ * (c - 1) mod order (-1, 0, 1, ...), which no LAK kernel accepts as
+ * positions; the fill here is the position the rest of this API uses. */
+int slak_mesh_unreduce(slak_mesh* mesh, void* stream);
+
 /* Exclusive prefix sum with total: out[i] = sum(in[0..i)), out[n] = sum(in)
  * (CubFunctions::scan_exclusive, LibInclude/device/cub_wrappers.cuh:30-43,
  * which also writes n+1 values). Device pointers; out holds n+1 int32. */

@@ -109,6 +109,8 @@ EXPORTS = [
     ("slak_subdivide", ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SlakMesh), ctypes.POINTER(SlakMesh),
                                       ctypes.c_int32, ctypes.POINTER(SlakStats), ctypes.c_void_p]),
     ("slak_free", ctypes.c_int, [ctypes.c_void_p]),
+    ("slak_mesh_reduce", ctypes.c_int, [ctypes.POINTER(SlakMesh), ctypes.c_void_p]),
+    ("slak_mesh_unreduce", ctypes.c_int, [ctypes.POINTER(SlakMesh), ctypes.c_void_p]),
     ("slak_last_error", ctypes.c_char_p, []),
     ("slak_version", ctypes.c_int32, []),
     ("slak_malloc", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint64]),

@@ -61,6 +61,12 @@ void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st);
 // LAK form (ptr, vals) of every segment of a level with dst_ptr set; nnz =
 // the level's union corners (= refined faces)
 void launch_lak_form(const SegTable& seg, int64_t nnz, hipStream_t st);
+// homogeneous-mesh conversions: per-face reorder of (ids, vals) by vals or by
+// ids (faces by ptr, or stride when ptr is null; 3 or 4 corners); fills of
+// stride ptr / 1-based positions
+void launch_sort_faces(bool by_vals, const int32_t* ptr, int32_t stride, int32_t nf, int64_t nnz, int32_t* ids,
+                       int32_t* vals, int* flags, hipStream_t st);
+void launch_fill_homogeneous(int32_t* ptr, int64_t nptr, int32_t* vals, int64_t nvals, int32_t stride, hipStream_t st);
 int64_t scan_partials_len(int64_t n);
 // out[i] = sum(in[0..i)), out[n] = total (also to *total_out when given)
 void launch_scan(const int32_t* in, int32_t* out, int64_t n, int64_t* partial, int* flags, hipStream_t st,

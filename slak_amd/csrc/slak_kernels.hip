@@ -775,6 +775,58 @@ __global__ void __launch_bounds__(kBlock) k_lak_form(SegTable seg, int64_t nnz) 
 }
 
 // ---------------------------------------------------------------------------
+// reduced / unreduced conversions of homogeneous meshes (GPU::CSCMesh::reduce /
+// unreduce, include/Mesh.h:305-377): each face's (ids, vals) reordered by vals
+// (reduce: ids in corner-position order) or by ids (unreduce), in place.
+
+template <bool KEY_VALS>
+__global__ void __launch_bounds__(kBlock) k_sort_faces(const int32_t* __restrict__ ptr, int32_t stride, int32_t nf,
+                                                       int64_t nnz, int32_t* __restrict__ ids,
+                                                       int32_t* __restrict__ vals, int* __restrict__ flags) {
+  const int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
+  if (f >= nf) return;
+  const int64_t s = ptr ? ptr[f] : (int64_t)stride * f;
+  const int64_t e = ptr ? ptr[f + 1] : s + stride;
+  const int n = (int)(e - s);
+  if (s < 0 || e > nnz || n < 0 || n > 4) {  // homogeneous faces have 3 or 4 corners
+    atomicOr(flags, kFlagBadPtr);
+    return;
+  }
+  int32_t k[4], a[4], b[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (i < n) {
+      a[i] = ids[s + i];
+      b[i] = vals[s + i];
+      k[i] = KEY_VALS ? b[i] : a[i];
+    }
+  // insertion sort by key, carrying (id, val)
+#pragma unroll
+  for (int i = 1; i < 4; ++i)
+#pragma unroll
+    for (int j = i; j > 0; --j)
+      if (i < n && k[j - 1] > k[j]) {
+        int32_t x = k[j]; k[j] = k[j - 1]; k[j - 1] = x;
+        x = a[j]; a[j] = a[j - 1]; a[j - 1] = x;
+        x = b[j]; b[j] = b[j - 1]; b[j - 1] = x;
+      }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (i < n) {
+      ids[s + i] = a[i];
+      vals[s + i] = b[i];
+    }
+}
+
+// vals[i] = 1-based corner position (i mod stride) + 1; ptr[f] = stride f
+__global__ void k_fill_homogeneous(int32_t* __restrict__ ptr, int64_t nptr, int32_t* __restrict__ vals, int64_t nvals,
+                                   int32_t stride) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ptr && i < nptr) ptr[i] = (int32_t)(stride * i);
+  if (vals && i < nvals) vals[i] = (int32_t)(i % stride) + 1;
+}
+
+// ---------------------------------------------------------------------------
 // exclusive scan with total: out[i] = sum(in[0..i)), out[n] = sum(in)
 
 constexpr int kScanItems = 8;
@@ -1094,6 +1146,21 @@ void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st) {
 void launch_lak_form(const SegTable& seg, int64_t nnz, hipStream_t st) {
   if (nnz <= 0) return;
   k_lak_form<<<grid_for(nnz), kBlock, 0, st>>>(seg, nnz);
+}
+
+void launch_sort_faces(bool by_vals, const int32_t* ptr, int32_t stride, int32_t nf, int64_t nnz, int32_t* ids,
+                       int32_t* vals, int* flags, hipStream_t st) {
+  if (nf <= 0) return;
+  if (by_vals)
+    k_sort_faces<true><<<grid_for(nf), kBlock, 0, st>>>(ptr, stride, nf, nnz, ids, vals, flags);
+  else
+    k_sort_faces<false><<<grid_for(nf), kBlock, 0, st>>>(ptr, stride, nf, nnz, ids, vals, flags);
+}
+
+void launch_fill_homogeneous(int32_t* ptr, int64_t nptr, int32_t* vals, int64_t nvals, int32_t stride, hipStream_t st) {
+  const int64_t n = std::max(ptr ? nptr : 0, vals ? nvals : 0);
+  if (n <= 0) return;
+  k_fill_homogeneous<<<grid_for(n), kBlock, 0, st>>>(ptr, nptr, vals, nvals, stride);
 }
 
 int64_t scan_partials_len(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }

@@ -482,6 +482,49 @@ int slak_copy_probe(uint64_t bytes, int32_t iters, double* gbps) {
   });
 }
 
+static void convert_homogeneous(slak_mesh* m, bool reduce, hipStream_t st) {
+  if (!m) throw Error(SLAK_ERR_INVALID_ARGUMENT, "null mesh");
+  if (m->type == SLAK_POLY) throw Error(SLAK_ERR_INVALID_ARGUMENT, "mesh type not homogeneous");
+  if (m->type != SLAK_TRI && m->type != SLAK_QUAD) throw Error(SLAK_ERR_INVALID_ARGUMENT, "unknown mesh type");
+  if (reduce == (m->is_reduced != 0)) return;  // already in the requested form
+  const int32_t stride = m->type == SLAK_QUAD ? 4 : 3;
+  if (m->nnz > 0 && !m->ids) throw Error(SLAK_ERR_INVALID_ARGUMENT, "mesh without ids");
+  if ((int64_t)m->nnz != (int64_t)stride * (int64_t)m->nfaces)
+    throw Error(SLAK_ERR_INVALID_ARGUMENT, "homogeneous mesh with nnz != face order * nfaces");
+  if (m->nnz > 0x7fffffffull) throw Error(SLAK_ERR_INVALID_ARGUMENT, "nnz too large for int32 indices");
+  slak::Arena arena(slak::pool());
+  int* flags = arena.get_n<int>(1);
+  slak::launch_fill_i32(flags, 1, 0, st);
+  if (reduce) {
+    if (!m->vals) throw Error(SLAK_ERR_INVALID_ARGUMENT, "unreduced mesh without vals");
+  } else {
+    // ptr and vals of a reduced mesh may be absent: stride offsets and 1-based
+    // corner positions (the reference's unreduce fills vals with (c - 1) mod
+    // order, i.e. -1, 0, 1, ..., which no LAK kernel accepts as positions)
+    int32_t* ptr = nullptr;
+    int32_t* vals = nullptr;
+    if (!m->ptr) ptr = static_cast<int32_t*>(slak::pool().alloc(((size_t)m->nfaces + 1) * sizeof(int32_t)));
+    if (!m->vals) vals = static_cast<int32_t*>(slak::pool().alloc(std::max<size_t>(m->nnz, 1) * sizeof(int32_t)));
+    slak::launch_fill_homogeneous(ptr, (int64_t)m->nfaces + 1, vals, (int64_t)m->nnz, stride, st);
+    if (ptr) m->ptr = ptr;
+    if (vals) m->vals = vals;
+  }
+  slak::launch_sort_faces(reduce, m->ptr, stride, (int32_t)m->nfaces, (int64_t)m->nnz, m->ids, m->vals, flags, st);
+  int fl = 0;
+  SLAK_HIP_CHECK(hipMemcpyAsync(&fl, flags, sizeof(int), hipMemcpyDeviceToHost, st));
+  SLAK_HIP_CHECK(hipStreamSynchronize(st));
+  if (fl) throw Error(SLAK_ERR_INVALID_MESH, "invalid mesh: ptr does not describe faces of the mesh type's order");
+  m->is_reduced = reduce ? 1 : 0;
+}
+
+int slak_mesh_reduce(slak_mesh* mesh, void* stream) {
+  return guarded([&] { convert_homogeneous(mesh, true, static_cast<hipStream_t>(stream)); });
+}
+
+int slak_mesh_unreduce(slak_mesh* mesh, void* stream) {
+  return guarded([&] { convert_homogeneous(mesh, false, static_cast<hipStream_t>(stream)); });
+}
+
 int slak_scan_exclusive(const int32_t* in, int32_t* out, uint64_t n, void* stream) {
   return guarded([&] {
     if ((!in && n) || !out) throw Error(SLAK_ERR_INVALID_ARGUMENT, "null pointer");

@@ -116,6 +116,19 @@ class MeshInfo:
             b.ptr = 0
         return m
 
+    def reduce(self, stream: int = 0) -> None:
+        """GPU::CSCMesh::reduce (include/Mesh.h:305-334) on the device."""
+        m = self.to_c()
+        check(lib().slak_mesh_reduce(ctypes.byref(m), stream or None))
+        self.from_c(m)
+
+    def unreduce(self, stream: int = 0) -> None:
+        """GPU::CSCMesh::unreduce (include/Mesh.h:336-377) on the device; a
+        missing ptr / vals is allocated (freed by freeAndReset)."""
+        m = self.to_c()
+        check(lib().slak_mesh_unreduce(ctypes.byref(m), stream or None))
+        self.from_c(m)
+
     def to_host(self) -> HostMesh:
         ptr = to_host(self.ptr, self.nfaces + 1, np.int32) if self.ptr else None
         vals = to_host(self.vals, self.nnz, np.int32) if self.vals else None
