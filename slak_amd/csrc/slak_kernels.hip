@@ -390,6 +390,10 @@ __device__ __forceinline__ void store_nt(int32_t* p, int a, int b, int c, int d)
   __builtin_nontemporal_store(x, reinterpret_cast<v4i_t*>(p));
 }
 __device__ __forceinline__ void store_nt(int32_t* p, int a) { __builtin_nontemporal_store(a, p); }
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_nt(float4* p, float4 q) {
+  __builtin_nontemporal_store(__builtin_bit_cast(v4f_t, q), reinterpret_cast<v4f_t*>(p));
+}
 
 // Per-segment constants of a refine thread.
 struct SegK {
@@ -485,7 +489,8 @@ __device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K,
       q = f4mul(f4add(f4add(f4add(c.plo, c.phi), c.fa), c.fz), 0.25f);
     else
       q = f4mul(f4add(c.plo, c.phi), 0.5f);
-    S->dst_verts[K.snv + K.snf + c.e1] = q;
+    if (final_level) store_nt(S->dst_verts + (K.snv + K.snf + c.e1), q);
+    else S->dst_verts[K.snv + K.snf + c.e1] = q;
   }
   if (!K.next) return;
 
@@ -715,8 +720,11 @@ __device__ __forceinline__ void vertex_item(const RefineView& R, int32_t v) {
     for (int i = 0; i < n; ++i)
       if (ci[prevc(row[i].x)].y < 0) q = f4add(q, f4mul(P[row[i].y], 0.125f));
   }
+  if (!next) {
+    store_nt(dst + (v - vb), q);
+    return;
+  }
   dst[v - vb] = q;
-  if (!next) return;
 
   const int32_t ncb = (int32_t)S->next_cb, nvb = (int32_t)S->next_vb;
   const int32_t nep = nvb + snv + S->nf;
