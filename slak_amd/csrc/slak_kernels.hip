@@ -863,80 +863,96 @@ __device__ __forceinline__ int64_t block_exclusive_scan(int64_t x, int64_t* lds,
 }
 
 // Inputs up to kScanSmall items: one block of 1024 threads in one launch
-// (instead of three). Chunks of 8192 items are loaded coalesced into
-// registers one chunk ahead, staged through LDS (padded by one word per 32 so
-// that each thread's 8 contiguous items sit in distinct banks), scanned per
-// thread and across the block, with a running carry.
+// (instead of three).
 constexpr int kScanSmallThreads = 1024;
-constexpr int kScanSmallItems = 8;
-constexpr int kScanSmallChunk = kScanSmallThreads * kScanSmallItems;
-constexpr int64_t kScanSmall = 4 * (int64_t)kScanSmallChunk;
+constexpr int kScanSmallRounds = 8;  // int4 loads per thread
+constexpr int kScanSmallRound = 4 * kScanSmallThreads;
+constexpr int64_t kScanSmall = (int64_t)kScanSmallRounds * kScanSmallRound;
 
-__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
+// 64-bit lane exchange by DPP (row_shr / row_bcast); lanes without a source,
+// and rows outside ROWMASK, read 0.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int64_t dpp64(int64_t x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)x, CTRL, ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, ROWMASK, 0xF, false);
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+// Inclusive scan over the 64 lanes of a wave: shifts by 1, 2, 4, 8 inside
+// each row of 16 lanes, then lane 15 of rows 0 / 2 into rows 1 / 3 and lane
+// 31 into rows 2 and 3.
+__device__ __forceinline__ int64_t wave_scan64(int64_t x) {
+  x += dpp64<0x111, 0xF>(x);
+  x += dpp64<0x112, 0xF>(x);
+  x += dpp64<0x114, 0xF>(x);
+  x += dpp64<0x118, 0xF>(x);
+  x += dpp64<0x142, 0xA>(x);
+  x += dpp64<0x143, 0xC>(x);
+  return x;
+}
 
+// Item (r, tid, j) = r * 4096 + 4 tid + j: every load and store instruction
+// of a wave covers 1 KB of consecutive ints, all loads are in flight before
+// the first add, the eight rounds are scanned side by side in registers, and
+// wave 0 scans the 128 wave totals (rounds major) between two barriers.
 __global__ void __launch_bounds__(kScanSmallThreads) k_scan_small(const int32_t* __restrict__ in, int64_t n,
                                                                   int32_t* __restrict__ out, int* __restrict__ flags,
                                                                   int32_t* __restrict__ total_out) {
-  __shared__ int32_t buf[kScanSmallChunk + kScanSmallChunk / 32];
-  __shared__ int64_t wsum[kScanSmallThreads / 64];
+  constexpr int kWaves = kScanSmallThreads / 64;
+  static_assert(kScanSmallRounds * kWaves == 128, "wave 0 scans two wave totals per lane");
+  __shared__ int64_t wsum[kScanSmallRounds * kWaves];
+  __shared__ int64_t total;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // chunk c + 1 is loaded (coalesced, into registers) while chunk c is scanned
-  int32_t cur[kScanSmallItems], nxt[kScanSmallItems];
+  const bool vin = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+  const bool vout = (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+  int4 x[kScanSmallRounds];
 #pragma unroll
-  for (int k = 0; k < kScanSmallItems; ++k) {
-    const int64_t i = k * kScanSmallThreads + tid;
-    cur[k] = i < n ? in[i] : 0;
+  for (int r = 0; r < kScanSmallRounds; ++r) {
+    const int64_t i = (int64_t)r * kScanSmallRound + 4 * tid;
+    if (vin && i + 3 < n) {
+      x[r] = *reinterpret_cast<const int4*>(in + i);
+    } else {
+      x[r].x = i < n ? in[i] : 0;
+      x[r].y = i + 1 < n ? in[i + 1] : 0;
+      x[r].z = i + 2 < n ? in[i + 2] : 0;
+      x[r].w = i + 3 < n ? in[i + 3] : 0;
+    }
   }
-  int64_t carry = 0;
-  for (int64_t base = 0; base < n; base += kScanSmallChunk) {
+  int64_t g[kScanSmallRounds], v[kScanSmallRounds];
 #pragma unroll
-    for (int k = 0; k < kScanSmallItems; ++k) {
-      const int64_t i = base + kScanSmallChunk + k * kScanSmallThreads + tid;
-      nxt[k] = i < n ? in[i] : 0;
+  for (int r = 0; r < kScanSmallRounds; ++r) {
+    g[r] = (int64_t)x[r].x + x[r].y + x[r].z + x[r].w;
+    v[r] = wave_scan64(g[r]);
+  }
+  if (lane == 63) {
+#pragma unroll
+    for (int r = 0; r < kScanSmallRounds; ++r) wsum[r * kWaves + wave] = v[r];
+  }
+  __syncthreads();
+  if (wave == 0) {  // 128 wave totals, two per lane: exclusive prefixes in place
+    const int64_t a = wsum[2 * lane], b = wsum[2 * lane + 1];
+    const int64_t inc = wave_scan64(a + b);
+    wsum[2 * lane] = inc - a - b;
+    wsum[2 * lane + 1] = inc - b;
+    if (lane == 63) total = inc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kScanSmallRounds; ++r) {
+    const int64_t e0 = wsum[r * kWaves + wave] + v[r] - g[r];
+    const int4 o = make_int4((int32_t)e0, (int32_t)(e0 + x[r].x), (int32_t)(e0 + x[r].x + x[r].y),
+                             (int32_t)(e0 + x[r].x + x[r].y + x[r].z));
+    const int64_t i = (int64_t)r * kScanSmallRound + 4 * tid;
+    if (vout && i + 3 < n) {
+      *reinterpret_cast<int4*>(out + i) = o;
+    } else {
+      if (i < n) out[i] = o.x;
+      if (i + 1 < n) out[i + 1] = o.y;
+      if (i + 2 < n) out[i + 2] = o.z;
+      if (i + 3 < n) out[i + 3] = o.w;
     }
-#pragma unroll
-    for (int k = 0; k < kScanSmallItems; ++k) buf[lds_pad(k * kScanSmallThreads + tid)] = cur[k];
-    __syncthreads();
-    int32_t x[kScanSmallItems];
-    int64_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanSmallItems; ++k) {
-      x[k] = buf[lds_pad(kScanSmallItems * tid + k)];
-      s += x[k];
-    }
-    int64_t v = s;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int64_t y = __shfl_up(v, d, 64);
-      if (lane >= d) v += y;
-    }
-    if (lane == 63) wsum[wave] = v;
-    __syncthreads();
-    int64_t off = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < kScanSmallThreads / 64; ++w) {
-      const int64_t t = wsum[w];
-      off += (w < wave) ? t : 0;
-      tot += t;
-    }
-    int64_t run = carry + off + v - s;
-#pragma unroll
-    for (int k = 0; k < kScanSmallItems; ++k) {
-      buf[lds_pad(kScanSmallItems * tid + k)] = (int32_t)run;
-      run += x[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kScanSmallItems; ++k) {
-      const int i = k * kScanSmallThreads + tid;
-      if (base + i < n) out[base + i] = buf[lds_pad(i)];
-    }
-    carry += tot;
-#pragma unroll
-    for (int k = 0; k < kScanSmallItems; ++k) cur[k] = nxt[k];
-    __syncthreads();
   }
   if (tid == 0) {
+    const int64_t carry = total;
     if (carry > 0x7fffffffll) atomicOr(flags, kFlagOverflow);
     out[n] = (int32_t)carry;
     if (total_out) *total_out = (int32_t)carry;

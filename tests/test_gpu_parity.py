@@ -168,6 +168,23 @@ def test_scan_operator(n):
     np.testing.assert_array_equal(y, ref)
 
 
+@pytest.mark.parametrize("shift", [(1, 0), (0, 3), (2, 1)], ids=["in+1", "out+3", "in+2,out+1"])
+@pytest.mark.parametrize("n", [5, 4099, 32768])
+def test_scan_operator_unaligned(n, shift):
+    """Pointers off the 16-byte grid take the per-item path of the one-block
+    scan; the results are the same."""
+    from slak_amd.mesh import DeviceBuffer, to_host
+    si, so = shift
+    rng = np.random.default_rng(n + si)
+    x = rng.integers(0, 1000, size=n).astype(np.int32)
+    din = DeviceBuffer.from_array(np.concatenate([np.zeros(si, np.int32), x]))
+    dout = DeviceBuffer(4 * (n + 1 + so))
+    capi.check(capi.lib().slak_scan_exclusive(din.ptr + 4 * si, dout.ptr + 4 * so, n, None))
+    y = to_host(dout.ptr + 4 * so, n + 1, np.int32)
+    ref = np.concatenate([[0], np.cumsum(x, dtype=np.int64)]).astype(np.int32)
+    np.testing.assert_array_equal(y, ref)
+
+
 # ---------------------------------------------------------------------------
 # boundary behaviour of the API
 
