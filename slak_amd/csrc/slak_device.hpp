@@ -65,6 +65,7 @@ struct SegDesc {
   const float4* src_verts;
   int32_t src_stride;
   int32_t require_quad;
+  int32_t dst_final;       // 1: the destination is the caller's output (stored non-temporally)
 };
 
 struct SegTable {
@@ -443,6 +444,23 @@ struct RefineView {
   int32_t* nmpos;        // next level's M^T positions (SLAK)
   int32_t next_nv;       // next level's union vertices and corners (end of its M^T)
   int32_t next_nnz;
+};
+
+// The final level of a single mesh, computed from the structure of the quad
+// mesh M two levels before the output and the positions of its refinement M1
+// (whose structure is never written).
+struct FinalView {
+  const int32_t* ids;   // M corners -> vertices
+  const int2* ci;       // M: (edge id, twin corner or -1) per corner
+  const int32_t* tptr;  // M: M^T row offsets
+  const int2* trow;     // M: M^T rows (corner, in-neighbour), sorted
+  const int32_t* blk;   // M: exclusive scan of its boundary edge flags, or null (closed)
+  const int32_t* mpos;  // M: M^T position per corner (SLAK)
+  int32_t nv, nf, ne;   // M sizes
+  int32_t nnz;
+  const float4* P1;     // positions of M1 = [M vertices | M face points | M edge points]
+  float4* out_verts;    // output positions: [M1 vertices | M1 face points (already written) | M1 edge points]
+  int32_t* out_ids;     // output quads: 4 per M1 corner, 16 per M corner
 };
 
 // Face, successor and predecessor corner of corner t (quad: bit arithmetic;

@@ -477,7 +477,7 @@ __device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K,
   const SegDesc* __restrict__ S = K.S;
   const int32_t lt = c.t - K.cb, lf = c.f - K.fb;
   const int32_t epb = K.snv + K.snf + K.off, fpid = K.snv + lf + K.off;
-  const bool final_level = !K.next;
+  const bool final_level = S->dst_final != 0;  // the caller's output: not read again here
   if (final_level) {
     store_nt(S->dst_ids + 4 * lt, c.v - K.vb + K.off, epb + c.e1, fpid, epb + c.e0);
   } else {
@@ -720,11 +720,12 @@ __device__ __forceinline__ void vertex_item(const RefineView& R, int32_t v) {
     for (int i = 0; i < n; ++i)
       if (ci[prevc(row[i].x)].y < 0) q = f4add(q, f4mul(P[row[i].y], 0.125f));
   }
-  if (!next) {
+  if (!next && S->dst_final) {
     store_nt(dst + (v - vb), q);
     return;
   }
   dst[v - vb] = q;
+  if (!next) return;
 
   const int32_t ncb = (int32_t)S->next_cb, nvb = (int32_t)S->next_vb;
   const int32_t nep = nvb + snv + S->nf;
@@ -763,6 +764,274 @@ __global__ void __launch_bounds__(kBlock) k_level(RefineView R, int32_t nbr, int
     vertex_item<QUAD, SCHEME, NEXT, BND>(R, (int32_t)(before * blockDim.x + threadIdx.x));
   else
     refine_item<QUAD, SCHEME>(R, (b - before) * blockDim.x + threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Final level of a single mesh in closed form from the quad mesh M two levels
+// before the output. M1 (the refinement of M, whose positions P1 and face
+// points are in hand) has corners 4t + k for the corners t of M:
+//   quad t of M1   (v, ep(e1), fp(F), ep(e0))  in M1 vertex ids
+//   M1 edge ids    corner 4t + k leaves along E1[k] = (ex, ey, ez, ew) and
+//                  enters along E0[k] = (ew, ex, ey, ez), the ids the refine
+//                  items give M1 (LAK: 4e - bl(e) + rank, SLAK: M^T positions)
+//   output quad    of M1 corner 4t + k: (vertex, ep'(E1[k]), fp'(t), ep'(E0[k]))
+// One lane per corner t of M writes its four output quads (64 contiguous
+// bytes). The owner of edge e1 of M (t < twin, or no twin) writes the edge
+// points of the three or four M1 edges around ep(e1) (both halves of e1 and
+// the edges to fp(F) and fp(G)), contiguous for LAK, and the vertex point of
+// ep(e1), from the same nine positions. Lane 0 of each quad writes the vertex
+// point of fp(F) from its four lanes. Vertex items remain only for the
+// vertices of M. Every sum adds the same terms in the same order as the
+// refine and vertex items of M1 would: the results are the same bits.
+
+// DPP quad permutations: every lane of the quad reads lane K / lane i+1.
+template <int K>
+__device__ __forceinline__ float4 quad_lane(float4 x) {
+  constexpr int ctrl = K * 0x55;
+  return make_float4(__int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.x), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.y), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.z), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.w), ctrl, 0xF, 0xF, false)));
+}
+__device__ __forceinline__ float4 quad_next(float4 x) {
+  constexpr int ctrl = 0x39;  // [1, 2, 3, 0]
+  return make_float4(__int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.x), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.y), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.z), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.w), ctrl, 0xF, 0xF, false)));
+}
+__device__ __forceinline__ float4 f4sel(bool c, float4 a, float4 b) { return c ? a : b; }
+
+template <int SCHEME, bool BND>
+__device__ __forceinline__ void final_corner(const FinalView& V, int32_t t) {
+  // every lane of a quad runs (nnz = 4 nf and blocks hold whole quads)
+  const int32_t F = t >> 2, q0 = t & ~3;
+  const int32_t tn = q0 | ((t + 1) & 3), tp = q0 | ((t + 3) & 3);
+  const int32_t* __restrict__ ids = V.ids;
+  const int32_t v = ids[t], w = ids[tn], u = ids[tp];
+  const int2 c1 = V.ci[t], c0 = V.ci[tp];
+  const int32_t emax = V.ne > 0 ? V.ne - 1 : 0;
+  const int32_t e1 = clampi(c1.x, emax), e0 = clampi(c0.x, emax);
+  const int32_t s1 = c1.y, s0 = c0.y;
+  const int32_t g1 = s1 >= 0 ? (s1 >> 2) : F, g0 = s0 >= 0 ? (s0 >> 2) : F;
+  const int32_t n1 = s1 >= 0 ? ((s1 & ~3) | ((s1 + 1) & 3)) : 0;  // next(s1)
+  const bool own = s1 < 0 || t < s1;
+  const int32_t nv = V.nv, nnz = V.nnz;
+  const int32_t ep0 = nv + V.nf;     // M1 vertex of the edge point of edge 0 of M
+  const int32_t nv1 = ep0 + V.ne;    // M1 vertices
+  const int32_t epb2 = nv1 + nnz;    // output vertex of the edge point of M1 edge 0
+  const float4* __restrict__ P1 = V.P1;
+  float4* __restrict__ out = V.out_verts;
+  const float4* __restrict__ fpp = out + nv1;  // face points of the M1 faces (= corners of M)
+  int32_t bl1 = 0, bl0 = 0;
+  if (BND) {
+    bl1 = V.blk[e1];
+    bl0 = V.blk[e0];
+  }
+  int32_t mp_t = 0, mp_s0 = 0, mp_s1 = 0;
+  if (SCHEME == kSLAK) {
+    mp_t = V.mpos[t];
+    if (s0 >= 0) mp_s0 = V.mpos[s0];
+    if (own && s1 >= 0) mp_s1 = V.mpos[s1];
+  }
+  const float4 Pe = P1[ep0 + e1];   // ep(e1)
+  const float4 Pv = P1[v];
+  const float4 PF = P1[nv + F];     // fp(F)
+  const float4 fpt = fpp[t];
+  float4 PG = make_float4(0.f, 0.f, 0.f, 0.f), fps = PG, fpsn = PG;
+  if (own && s1 >= 0) {
+    PG = P1[nv + g1];
+    fps = fpp[s1];
+    fpsn = fpp[n1];
+  }
+  // from the other lanes of the quad (all four run this)
+  const float4 Pw = quad_next(Pv), fptn = quad_next(fpt);
+  const float4 fq1 = quad_lane<1>(fpt), fq2 = quad_lane<2>(fpt), fq3 = quad_lane<3>(fpt);
+  const float4 pq1 = quad_lane<1>(Pe), pq2 = quad_lane<2>(Pe), pq3 = quad_lane<3>(Pe);
+
+  // M1 edge ids of the corners 4t .. 4t+3 (as the refine items number M1)
+  int32_t ex, ey, ez, ew;
+  if (SCHEME == kLAK) {
+    const int32_t base1 = 4 * e1 - bl1, base0 = 4 * e0 - bl0;
+    ex = base1 + (v > w ? 1 : 0);
+    ey = base1 + 2 + ((s1 >= 0 && F > g1) ? 1 : 0);
+    ez = base0 + 2 + ((s0 >= 0 && F > g0) ? 1 : 0);
+    ew = base0 + (v > u ? 1 : 0);
+  } else {
+    ex = mp_t;
+    ey = nnz + tn;
+    ez = nnz + t;
+    ew = s0 < 0 ? 2 * nnz + bl0 : mp_s0;
+  }
+  // the four output quads of M1 face t
+  {
+    const int32_t x1 = ep0 + e1, x2 = nv + F, x3 = ep0 + e0, fid = nv1 + t;
+    // cached stores: the four 16-byte pieces of a lane complete each other's
+    // lines in L2 (non-temporal stores of partial lines cost several times more)
+    int4* __restrict__ o = reinterpret_cast<int4*>(V.out_ids) + 4 * (int64_t)t;
+    o[0] = make_int4(v, epb2 + ex, fid, epb2 + ew);
+    o[1] = make_int4(x1, epb2 + ey, fid, epb2 + ex);
+    o[2] = make_int4(x2, epb2 + ez, fid, epb2 + ey);
+    o[3] = make_int4(x3, epb2 + ew, fid, epb2 + ez);
+  }
+  // vertex point of fp(F): row 4t'+2 for the corners t' of F in order,
+  // in-neighbours ep(e1(t')); never on the boundary
+  if ((t & 3) == 0) {
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    s = f4add(s, fpt);
+    s = f4add(s, fq1);
+    s = f4add(s, fq2);
+    s = f4add(s, fq3);
+    s = f4add(s, Pe);
+    s = f4add(s, pq1);
+    s = f4add(s, pq2);
+    s = f4add(s, pq3);
+    out[nv + F] = f4div(f4add(f4div(s, 4.0f), f4mul(PF, 2.0f)), 4.0f);
+  }
+  if (!own) return;
+  // edge points of the M1 edges around ep(e1), by lower end point then the
+  // faces of the half-edges lower -> upper and upper -> lower
+  const bool inner = !BND || s1 >= 0;
+  const float4 q_vh = inner ? f4mul(f4add(f4add(f4add(Pv, Pe), fpt), fpsn), 0.25f) : f4mul(f4add(Pv, Pe), 0.5f);
+  const float4 q_wh = inner ? f4mul(f4add(f4add(f4add(Pw, Pe), fps), fptn), 0.25f) : f4mul(f4add(Pw, Pe), 0.5f);
+  const float4 q_fF = f4mul(f4add(f4add(f4add(PF, Pe), fptn), fpt), 0.25f);
+  int32_t i_vh, i_wh, i_fF, i_fG;
+  if (SCHEME == kLAK) {
+    const int32_t base1 = 4 * e1 - bl1;
+    i_vh = base1 + (v > w ? 1 : 0);
+    i_wh = base1 + (w > v ? 1 : 0);
+    i_fF = base1 + 2 + ((s1 >= 0 && F > g1) ? 1 : 0);
+    i_fG = base1 + 2 + (g1 > F ? 1 : 0);
+  } else {
+    i_vh = mp_t;
+    i_wh = s1 >= 0 ? mp_s1 : 2 * nnz + bl1;
+    i_fF = nnz + tn;
+    i_fG = nnz + n1;
+  }
+  // LAK: one 64-byte group, cached; SLAK: four scattered points, non-temporal
+  float4* __restrict__ eo = out + epb2;
+  const float4 q_fG = f4mul(f4add(f4add(f4add(PG, Pe), fpsn), fps), 0.25f);
+  if (SCHEME == kLAK) {
+    eo[i_vh] = q_vh;
+    eo[i_wh] = q_wh;
+    eo[i_fF] = q_fF;
+    if (inner) eo[i_fG] = q_fG;
+  } else {
+    store_nt(eo + i_vh, q_vh);
+    store_nt(eo + i_wh, q_wh);
+    store_nt(eo + i_fF, q_fF);
+    if (inner) store_nt(eo + i_fG, q_fG);
+  }
+  // vertex point of ep(e1)
+  float4 q;
+  if (inner) {
+    // row: corners 4t+1 (face t, in-neighbour v), 4tn+3 (tn, fp(F)),
+    // 4s1+1 (s1, w), 4n1+3 (n1, fp(G)), sorted
+    const int32_t ka = 4 * t + 1, kb = 4 * tn + 3, kc = 4 * s1 + 1, kd = 4 * n1 + 3;
+    const int ra = (kb < ka) + (kc < ka) + (kd < ka);
+    const int rb = (ka < kb) + (kc < kb) + (kd < kb);
+    const int rc = (ka < kc) + (kb < kc) + (kd < kc);
+    float4 f[4], p[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      f[i] = f4sel(ra == i, fpt, f4sel(rb == i, fptn, f4sel(rc == i, fps, fpsn)));
+      p[i] = f4sel(ra == i, Pv, f4sel(rb == i, PF, f4sel(rc == i, Pw, PG)));
+    }
+    float4 sm = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) sm = f4add(sm, f[i]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) sm = f4add(sm, p[i]);
+    q = f4div(f4add(f4div(sm, 4.0f), f4mul(Pe, 2.0f)), 4.0f);
+  } else {
+    q = f4add(f4add(f4mul(Pe, 0.75f), f4mul(Pw, 0.125f)), f4mul(Pv, 0.125f));
+  }
+  out[ep0 + e1] = q;
+}
+
+// vertex point of a vertex v of M in the output: row 4t for its corners t in
+// order, faces t, in-neighbours ep(e0(t)); on the boundary 3/4 P1[v] plus 1/8
+// of ep(e1(t)) across every boundary out-edge and of ep(e0(t)) across every
+// boundary in-edge
+template <int SCHEME, bool BND>
+__device__ __forceinline__ void final_vertex(const FinalView& V, int32_t v) {
+  if (v >= V.nv) return;
+  const int32_t ep0 = V.nv + V.nf, nv1 = ep0 + V.ne;
+  const int32_t emax = V.ne > 0 ? V.ne - 1 : 0;
+  const float4* __restrict__ P1 = V.P1;
+  const float4* __restrict__ fpp = V.out_verts + nv1;
+  const int2* __restrict__ ci = V.ci;
+  const int32_t p0 = V.tptr[v];
+  const int n = V.tptr[v + 1] - p0;
+  const int2* __restrict__ row = V.trow + p0;
+  const float4 Pv = P1[v];
+  auto prevc = [](int32_t t) __attribute__((always_inline)) { return (t & ~3) | ((t + 3) & 3); };
+  int32_t tt[kVertexRegs];
+  int2 cp[kVertexRegs], co[kVertexRegs];
+  float4 a[kVertexRegs];
+#pragma unroll
+  for (int i = 0; i < kVertexRegs; ++i)
+    if (i < n) tt[i] = row[i].x;
+#pragma unroll
+  for (int i = 0; i < kVertexRegs; ++i)
+    if (i < n) {
+      a[i] = fpp[tt[i]];
+      cp[i] = ci[prevc(tt[i])];
+      if (BND) co[i] = ci[tt[i]];
+    }
+  bool boundary = false;
+  if (BND) {
+#pragma unroll
+    for (int i = 0; i < kVertexRegs; ++i)
+      if (i < n) boundary |= co[i].y < 0 || cp[i].y < 0;
+    for (int i = kVertexRegs; i < n; ++i) {
+      const int32_t t = row[i].x;
+      boundary |= ci[t].y < 0 || ci[prevc(t)].y < 0;
+    }
+  }
+  float4 q;
+  if (!boundary) {
+    float4 b[kVertexRegs];
+#pragma unroll
+    for (int i = 0; i < kVertexRegs; ++i)
+      if (i < n) b[i] = P1[ep0 + clampi(cp[i].x, emax)];
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < kVertexRegs; ++i)
+      if (i < n) s = f4add(s, a[i]);
+    for (int i = kVertexRegs; i < n; ++i) s = f4add(s, fpp[row[i].x]);
+#pragma unroll
+    for (int i = 0; i < kVertexRegs; ++i)
+      if (i < n) s = f4add(s, b[i]);
+    for (int i = kVertexRegs; i < n; ++i) s = f4add(s, P1[ep0 + clampi(ci[prevc(row[i].x)].x, emax)]);
+    const float nf = (float)n;
+    q = f4div(f4add(f4div(s, nf), f4mul(Pv, nf - 2.0f)), nf);
+  } else {
+    q = f4mul(Pv, 0.75f);
+    for (int i = 0; i < n; ++i) {
+      const int2 c = ci[row[i].x];
+      if (c.y < 0) q = f4add(q, f4mul(P1[ep0 + clampi(c.x, emax)], 0.125f));
+    }
+    for (int i = 0; i < n; ++i) {
+      const int2 c = ci[prevc(row[i].x)];
+      if (c.y < 0) q = f4add(q, f4mul(P1[ep0 + clampi(c.x, emax)], 0.125f));
+    }
+  }
+  V.out_verts[v] = q;
+}
+
+// corner items and vertex items interleaved in proportion, as in k_level
+template <int SCHEME, bool BND>
+__global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int32_t nbv) {
+  const int64_t b = block_index();
+  const int64_t T = (int64_t)nbc + nbv;
+  const int64_t before = b * nbv / T, after = (b + 1) * nbv / T;
+  if (after > before) {
+    final_vertex<SCHEME, BND>(V, (int32_t)(before * blockDim.x + threadIdx.x));
+  } else {
+    const int64_t t = (b - before) * blockDim.x + threadIdx.x;
+    if (t < V.nnz) final_corner<SCHEME, BND>(V, (int32_t)t);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1164,6 +1433,22 @@ void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st) {
   } else {
     if (scheme == kLAK) launch_level_t<false, kLAK>(R, g, nbr, nbv, st);
     else launch_level_t<false, kSLAK>(R, g, nbr, nbv, st);
+  }
+}
+
+void launch_final(int scheme, const FinalView& V, hipStream_t st) {
+  if (V.nnz <= 0 || (V.nnz & 3) || (int64_t)V.nnz != 4 * (int64_t)V.nf)
+    throw Error(5, "final level: the mesh two levels back must be quads");  // SLAK_ERR_INTERNAL
+  const int64_t nbc = grid_for(V.nnz), nbv = V.nv > 0 ? grid_for(V.nv) : 0;
+  if ((nbc + nbv) * kBlock > 0xffffffffll) throw Error(1, "level too large for one launch");
+  const unsigned g = (unsigned)(nbc + nbv);
+  const bool bnd = V.blk != nullptr;
+  if (scheme == kLAK) {
+    if (bnd) k_final<kLAK, true><<<g, kBlock, 0, st>>>(V, (int32_t)nbc, (int32_t)nbv);
+    else k_final<kLAK, false><<<g, kBlock, 0, st>>>(V, (int32_t)nbc, (int32_t)nbv);
+  } else {
+    if (bnd) k_final<kSLAK, true><<<g, kBlock, 0, st>>>(V, (int32_t)nbc, (int32_t)nbv);
+    else k_final<kSLAK, false><<<g, kBlock, 0, st>>>(V, (int32_t)nbc, (int32_t)nbv);
   }
 }
 

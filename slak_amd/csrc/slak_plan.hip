@@ -521,6 +521,7 @@ void Plan::prepare() {
         S.next_vb = -1;
         S.next_cb = -1;
         S.next_eb = -1;
+        S.dst_final = 1;
       } else {
         S.dst_verts = Un_P + vbn;
         S.dst_ids = Un_ids + cbn;
@@ -544,19 +545,36 @@ void Plan::prepare() {
       nx.P = Un_P;
     }
   }
+  fused_final_ = jobs_.size() == 1 && Lmax >= 2 && levels_[Lmax - 2].quad && levels_[Lmax - 1].segs.size() == 1;
+  if (const char* e = std::getenv("SLAK_FUSED_FINAL")) fused_final_ &= std::atoi(e) != 0;
+  if (fused_final_)
+    for (SegDesc& S : levels_[Lmax - 2].segs) {  // the level before the last writes no structure
+      S.next_vb = -1;
+      S.next_cb = -1;
+      S.next_eb = -1;
+    }
   // Levels >= 2 receive their structure from the level before (two
   // alternating buffers of each kind, sized for the largest level); the
   // boundary flags and their scan are one buffer pair shared by every level.
-  if (Lmax >= 2) {
-    int32_t* mt_ptr[2] = {arena_.get_n<int32_t>(max_nv + 1), Lmax >= 3 ? arena_.get_n<int32_t>(max_nv + 1) : nullptr};
-    int2* mt_row[2] = {arena_.get_n<int2>(max_nnz), Lmax >= 3 ? arena_.get_n<int2>(max_nnz) : nullptr};
-    int2* mt_ci[2] = {arena_.get_n<int2>(max_nnz), Lmax >= 3 ? arena_.get_n<int2>(max_nnz) : nullptr};
+  const int Lstruct = fused_final_ ? Lmax - 1 : Lmax;  // last level whose input structure is written
+  if (fused_final_) {
+    max_nv = max_nnz = 0;
+    for (int l = 2; l <= Lstruct; ++l) {
+      max_nv = std::max(max_nv, levels_[l - 1].nv);
+      max_nnz = std::max(max_nnz, levels_[l - 1].nnz);
+    }
+  }
+  if (Lstruct >= 2) {
+    int32_t* mt_ptr[2] = {arena_.get_n<int32_t>(max_nv + 1),
+                          Lstruct >= 3 ? arena_.get_n<int32_t>(max_nv + 1) : nullptr};
+    int2* mt_row[2] = {arena_.get_n<int2>(max_nnz), Lstruct >= 3 ? arena_.get_n<int2>(max_nnz) : nullptr};
+    int2* mt_ci[2] = {arena_.get_n<int2>(max_nnz), Lstruct >= 3 ? arena_.get_n<int2>(max_nnz) : nullptr};
     int32_t* mt_pos[2] = {nullptr, nullptr};
     if (scheme_ == kSLAK) {
       mt_pos[0] = arena_.get_n<int32_t>(max_nnz);
-      if (Lmax >= 3) mt_pos[1] = arena_.get_n<int32_t>(max_nnz);
+      if (Lstruct >= 3) mt_pos[1] = arena_.get_n<int32_t>(max_nnz);
     }
-    for (int l = 2; l <= Lmax; ++l) {
+    for (int l = 2; l <= Lstruct; ++l) {
       LevelPlan& L = levels_[l - 1];
       L.tptr = mt_ptr[l % 2];
       L.trow = mt_row[l % 2];
@@ -569,12 +587,12 @@ void Plan::prepare() {
     }
   }
   int64_t max_ne = 0;
-  for (auto& L : levels_)
-    if (L.boundary) max_ne = std::max(max_ne, L.ne);
+  for (int l = 1; l <= Lstruct; ++l)
+    if (levels_[l - 1].boundary) max_ne = std::max(max_ne, levels_[l - 1].ne);
   if (max_ne > 0) {
     int32_t* bnd = arena_.get_n<int32_t>(max_ne);
     int32_t* blk = arena_.get_n<int32_t>(max_ne + 1);
-    for (int l = 1; l <= Lmax; ++l) {
+    for (int l = 1; l <= Lstruct; ++l) {
       LevelPlan& L = levels_[l - 1];
       if (!L.boundary) continue;
       L.bnd = bnd;
@@ -636,6 +654,33 @@ void Plan::record_level(int l) {
     ProfScope ps(st_, L.quad ? "facepoints_quad" : "facepoints_poly",
                  4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + (L.quad ? 0.0 : 4.0 * (L.nf + 1)));
     launch_faces(L.quad, true, false, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), nullptr, nullptr, st_);
+  }
+  if (fused_final_ && l == (int)levels_.size()) {
+    const LevelPlan& M = levels_[l - 2];
+    FinalView V{};
+    V.ids = M.ids;
+    V.ci = M.ci;
+    V.tptr = M.tptr;
+    V.trow = M.trow;
+    V.blk = M.blk;
+    V.mpos = M.mpos;
+    V.nv = (int32_t)M.nv;
+    V.nf = (int32_t)M.nf;
+    V.ne = (int32_t)M.ne;
+    V.nnz = (int32_t)M.nnz;
+    V.P1 = L.P;
+    V.out_verts = L.segs[0].dst_verts;
+    V.out_ids = L.segs[0].dst_ids;
+    // algorithmic bytes: reads the ids, (edge, twin) pairs, M^T rows and
+    // offsets of M (boundary scan, SLAK positions), the positions of M1 and
+    // its face points; writes the output quads, vertex points and edge points
+    const double mne = (double)M.ne;
+    const double reads = 4.0 * M.nnz + 8.0 * M.nnz + 8.0 * M.nnz + 4.0 * M.nv + (M.blk ? 4.0 * mne : 0.0) +
+                         (scheme_ == kSLAK ? 4.0 * M.nnz : 0.0) + 16.0 * L.nv + 16.0 * L.nf;
+    const double writes = 16.0 * L.nnz + 16.0 * L.nv + 16.0 * ne;
+    ProfScope ps(st_, "refine_vertex_final_quad", reads + writes);
+    launch_final(scheme_, V, st_);
+    return;
   }
   RefineView R{};
   R.ids = L.ids;

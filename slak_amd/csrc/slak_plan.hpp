@@ -147,6 +147,10 @@ class Plan {
   hipEvent_t order_ev_ = nullptr;
   bool lak_forms_pending_ = false;
   bool lak_forms_in_place_ = false;  // written before a graph capture; replays keep them
+  // single mesh, two or more levels, quad mesh two levels before the output:
+  // the last level is computed from that mesh's structure (k_final) and the
+  // level before it writes no structure
+  bool fused_final_ = false;
   void launch_lak_forms();
   std::vector<void*> pooled_outputs_;
 };

@@ -184,3 +184,68 @@ def refine_structure(S: Structure, scheme: int) -> Structure:
 def b1_of(e: int, bl: int) -> int:
     """First edge id of the refined mesh owned by the edge point of e (LAK)."""
     return 4 * e - bl
+
+
+@dataclass
+class FinalForm:
+    ids2: np.ndarray                   # 16 per corner of M: the output quads of M1 face t
+    slots: dict                        # M1 edge id -> (owner corner of M, which edge)
+    fp_rows: List[List[Tuple[int, int]]]   # per face F of M: the M1 row of fp(F)
+    ep_rows: List[List[Tuple[int, int]]]   # per edge of M (owner order): the M1 row of ep(e)
+
+
+def final_closed_form(S: Structure, scheme: int) -> FinalForm:
+    """The last level in closed form from a quad mesh M, as the final-level
+    kernel (slak_amd/csrc/slak_kernels.hip, final_corner) computes it: the
+    output quads of every M1 corner, the M1 edge ids of the edge points each
+    owner of an M edge writes, and the M^T rows of the M1 vertices fp(F) and
+    ep(e) whose vertex points it sums in row order."""
+    ptr, ids, nv = S.ptr, S.ids, S.nv
+    nf, nnz, ne = len(ptr) - 1, len(ids), len(S.bnd)
+    assert nnz == 4 * nf
+    blk = np.concatenate([[0], np.cumsum(S.bnd)]).astype(np.int64)
+    ep0, nv1 = nv + nf, nv + nf + ne
+    epb2 = nv1 + nnz
+    ids2 = np.zeros(16 * nnz, dtype=np.int64)
+    slots = {}
+    fp_rows: List[List[Tuple[int, int]]] = [[] for _ in range(nf)]
+    ep_rows: List[List[Tuple[int, int]]] = []
+    for t in range(nnz):
+        F, q0 = t >> 2, t & ~3
+        tn, tp = q0 | ((t + 1) & 3), q0 | ((t + 3) & 3)
+        v, w, u = int(ids[t]), int(ids[tn]), int(ids[tp])
+        e1, e0 = int(S.eout[t]), int(S.eout[tp])
+        s1, s0 = int(S.twin[t]), int(S.twin[tp])
+        g1 = s1 >> 2 if s1 >= 0 else F
+        g0 = s0 >> 2 if s0 >= 0 else F
+        n1 = (s1 & ~3) | ((s1 + 1) & 3) if s1 >= 0 else 0
+        bl1, bl0 = int(blk[e1]), int(blk[e0])
+        if scheme == LAK:
+            b1, b0 = 4 * e1 - bl1, 4 * e0 - bl0
+            ex, ey = b1 + (v > w), b1 + 2 + (s1 >= 0 and F > g1)
+            ez, ew = b0 + 2 + (s0 >= 0 and F > g0), b0 + (v > u)
+        else:
+            ex, ey, ez = int(S.mpos[t]), nnz + tn, nnz + t
+            ew = 2 * nnz + bl0 if s0 < 0 else int(S.mpos[s0])
+        x = [v, ep0 + e1, nv + F, ep0 + e0]
+        E1 = [ex, ey, ez, ew]
+        E0 = [ew, ex, ey, ez]
+        for k in range(4):
+            ids2[16 * t + 4 * k: 16 * t + 4 * k + 4] = [x[k], epb2 + E1[k], nv1 + t, epb2 + E0[k]]
+        fp_rows[F].append((4 * t + 2, ep0 + e1))
+        if s1 < 0 or t < s1:
+            if scheme == LAK:
+                i_vh, i_wh = b1 + (v > w), b1 + (w > v)
+                i_fF, i_fG = b1 + 2 + (s1 >= 0 and F > g1), b1 + 2 + (g1 > F)
+            else:
+                i_vh = int(S.mpos[t])
+                i_wh = int(S.mpos[s1]) if s1 >= 0 else 2 * nnz + bl1
+                i_fF, i_fG = nnz + tn, nnz + n1
+            for name, i in (("vh", i_vh), ("wh", i_wh), ("fF", i_fF)) + ((("fG", i_fG),) if s1 >= 0 else ()):
+                assert i not in slots, (name, i)
+                slots[i] = (t, name)
+            ent = [(4 * t + 1, v), (4 * tn + 3, nv + F)]
+            if s1 >= 0:
+                ent += [(4 * s1 + 1, w), (4 * n1 + 3, nv + g1)]
+            ep_rows.append(sorted(ent))
+    return FinalForm(ids2, slots, fp_rows, ep_rows)
