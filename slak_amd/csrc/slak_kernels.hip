@@ -775,8 +775,8 @@ __global__ void __launch_bounds__(kBlock) k_level(RefineView R, int32_t nbr, int
 //                  enters along E0[k] = (ew, ex, ey, ez), the ids the refine
 //                  items give M1 (LAK: 4e - bl(e) + rank, SLAK: M^T positions)
 //   output quad    of M1 corner 4t + k: (vertex, ep'(E1[k]), fp'(t), ep'(E0[k]))
-// One lane per corner t of M writes its four output quads (64 contiguous
-// bytes). The owner of edge e1 of M (t < twin, or no twin) writes the edge
+// One lane per corner t of M computes its four output quads (64 contiguous
+// bytes, written out by the block through LDS). The owner of edge e1 of M (t < twin, or no twin) writes the edge
 // points of the three or four M1 edges around ep(e1) (both halves of e1 and
 // the edges to fp(F) and fp(G)), contiguous for LAK, and the vertex point of
 // ep(e1), from the same nine positions. Lane 0 of each quad writes the vertex
@@ -802,8 +802,10 @@ __device__ __forceinline__ float4 quad_next(float4 x) {
 }
 __device__ __forceinline__ float4 f4sel(bool c, float4 a, float4 b) { return c ? a : b; }
 
+constexpr int kStageStride = 5;  // int4 per lane in the staging area (4 used; 5 spreads the banks)
+
 template <int SCHEME, bool BND>
-__device__ __forceinline__ void final_corner(const FinalView& V, int32_t t) {
+__device__ __forceinline__ void final_corner(const FinalView& V, int32_t t, int4* __restrict__ stage) {
   // every lane of a quad runs (nnz = 4 nf and blocks hold whole quads)
   const int32_t F = t >> 2, q0 = t & ~3;
   const int32_t tn = q0 | ((t + 1) & 3), tp = q0 | ((t + 3) & 3);
@@ -866,9 +868,9 @@ __device__ __forceinline__ void final_corner(const FinalView& V, int32_t t) {
   // the four output quads of M1 face t
   {
     const int32_t x1 = ep0 + e1, x2 = nv + F, x3 = ep0 + e0, fid = nv1 + t;
-    // cached stores: the four 16-byte pieces of a lane complete each other's
-    // lines in L2 (non-temporal stores of partial lines cost several times more)
-    int4* __restrict__ o = reinterpret_cast<int4*>(V.out_ids) + 4 * (int64_t)t;
+    // into the block's staging area (the block writes them out as 1 KB runs:
+    // 64-byte pieces per lane, stored directly, cost several times more)
+    int4* __restrict__ o = stage + kStageStride * (threadIdx.x & (kBlock - 1));
     o[0] = make_int4(v, epb2 + ex, fid, epb2 + ew);
     o[1] = make_int4(x1, epb2 + ey, fid, epb2 + ex);
     o[2] = make_int4(x2, epb2 + ez, fid, epb2 + ey);
@@ -1020,7 +1022,9 @@ __device__ __forceinline__ void final_vertex(const FinalView& V, int32_t v) {
   V.out_verts[v] = q;
 }
 
-// corner items and vertex items interleaved in proportion, as in k_level
+// corner items and vertex items interleaved in proportion, as in k_level; a
+// corner block stages its 16 KB of output quads in LDS and stores them with
+// consecutive lanes on consecutive 16-byte pieces
 template <int SCHEME, bool BND>
 __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int32_t nbv) {
   const int64_t b = block_index();
@@ -1029,8 +1033,20 @@ __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int3
   if (after > before) {
     final_vertex<SCHEME, BND>(V, (int32_t)(before * blockDim.x + threadIdx.x));
   } else {
-    const int64_t t = (b - before) * blockDim.x + threadIdx.x;
-    if (t < V.nnz) final_corner<SCHEME, BND>(V, (int32_t)t);
+    __shared__ int4 stage[kStageStride * kBlock];
+    const int64_t t0 = (b - before) * blockDim.x, t = t0 + threadIdx.x;
+    if (t < V.nnz) final_corner<SCHEME, BND>(V, (int32_t)t, stage);
+    __syncthreads();  // the whole block: corner blocks are uniform
+    const int64_t nq = 4 * min((int64_t)kBlock, (int64_t)V.nnz - t0);
+    int32_t* __restrict__ dst = V.out_ids + 16 * t0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i = j * kBlock + threadIdx.x;
+      if (i < nq) {
+        const int4 x = stage[kStageStride * (i >> 2) + (i & 3)];
+        store_nt(dst + 4 * i, x.x, x.y, x.z, x.w);
+      }
+    }
   }
 }
 
