@@ -268,6 +268,7 @@ def serve_poisson(n_requests, qps_fraction, scheme, seed=5):
 
 # profiler stage -> rocprof kernel name prefix ({s}: scheme template argument)
 STAGE_KERNELS = {
+    "refine_vertex_final_quad": "k_final<{s},",
     "refine_vertex_quad": "k_level<true, {s},",
     "refine_vertex_poly": "k_level<false, {s},",
     "facepoints_quad": "k_faces<true, true, false>",
