@@ -53,14 +53,25 @@ def time_gpu(mesh, levels, scheme, steps, warmup, profile_steps=3):
     from slak_amd import capi
     from slak_amd.mesh import MeshInfo, _subdivide
 
+    import ctypes
     c = MeshInfo.from_host(mesh)
     faces = 0
     try:
+        # one library call per step (output allocation and free included),
+        # with the ctypes structures built once
+        lib = capi.lib()
+        cin = c.to_c()
+        cout = capi.SlakMesh()
+        stats = capi.SlakStats()
+        pin, pout, pst = ctypes.byref(cin), ctypes.byref(cout), ctypes.byref(stats)
+
         def step():
-            r = MeshInfo()
-            st = _subdivide(scheme, c, r, levels)
-            r.freeAndReset()
-            return st
+            capi.check(lib.slak_subdivide(scheme, pin, pout, levels, pst, None))
+            for p in (cout.ptr, cout.ids, cout.vals, cout.verts):
+                if p:
+                    capi.check(lib.slak_free(p))
+            cout.ptr = cout.ids = cout.vals = cout.verts = None
+            return stats
 
         for _ in range(warmup):
             step()

@@ -44,6 +44,40 @@ int64_t* pinned_buffer(size_t n) {
   return pin.p;
 }
 
+// Pinned host blocks for table uploads, by power-of-two size class (4 KiB
+// and up): a pageable source makes hipMemcpyAsync stage the bytes itself
+// before it returns, which delays every launch queued after it.
+std::mutex g_pin_mu;
+std::vector<std::pair<size_t, char*>> g_pin_free;
+
+char* pinned_acquire(size_t bytes, size_t& cap) {
+  cap = 4096;
+  while (cap < bytes) cap <<= 1;
+  {
+    std::lock_guard<std::mutex> g(g_pin_mu);
+    for (size_t i = 0; i < g_pin_free.size(); ++i)
+      if (g_pin_free[i].first == cap) {
+        char* p = g_pin_free[i].second;
+        g_pin_free[i] = g_pin_free.back();
+        g_pin_free.pop_back();
+        return p;
+      }
+  }
+  char* p = nullptr;
+  SLAK_HIP_CHECK(hipHostMalloc((void**)&p, cap));
+  return p;
+}
+
+void pinned_release(char* p, size_t cap) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(g_pin_mu);
+  if (g_pin_free.size() < 64) {
+    g_pin_free.emplace_back(cap, p);
+    return;
+  }
+  (void)hipHostFree(p);
+}
+
 void check_i32(int64_t x, const std::string& what) {
   if (x < 0 || x > 0x7fffffffll)
     throw Error(SLAK_ERR_INVALID_ARGUMENT, "too large for int32 indices: " + what + " = " + std::to_string(x));
@@ -93,7 +127,9 @@ Plan::~Plan() {
         for (void* p : {(void*)j.out_verts, (void*)j.out_ids, (void*)j.out_ptr, (void*)j.out_vals})
           if (p) pool().release(p);
   }
-  if (!arena_.empty()) (void)hipStreamSynchronize(st_);
+  if (!arena_.empty() || staging_first_.p || staging_all_.p) (void)hipStreamSynchronize(st_);
+  pinned_release(staging_first_.p, staging_first_.cap);
+  pinned_release(staging_all_.p, staging_all_.cap);
 }
 
 // Reads the flags word and the count ints after it in the state block in one
@@ -112,7 +148,7 @@ int Plan::readback(const int32_t* src, int count, int64_t* dst) {
 // Uploads the segment tables (SegDesc, vertex, face and corner bases) of levels
 // [l0, l1) in one copy from a plan-owned staging buffer (one per call site,
 // so a pending copy never sees its source rewritten).
-void Plan::upload_tables(int l0, int l1, std::vector<char>& staging, int64_t state_ints) {
+void Plan::upload_tables(int l0, int l1, Staging& staging, int64_t state_ints) {
   auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
   size_t total = 0;
   std::vector<size_t> offs;
@@ -124,7 +160,12 @@ void Plan::upload_tables(int l0, int l1, std::vector<char>& staging, int64_t sta
   const size_t state_off = total;
   total += align((size_t)state_ints * sizeof(int32_t));
   if (total == 0) return;
-  staging.assign(total, 0);
+  if (staging.p) SLAK_HIP_CHECK(hipStreamSynchronize(st_));  // reused: an earlier copy may still read it
+  if (staging.cap < total) {
+    pinned_release(staging.p, staging.cap);
+    staging.p = pinned_acquire(total, staging.cap);
+  }
+  std::memset(staging.p, 0, total);
   char* d = arena_.get_n<char>((int64_t)total);
   if (state_ints > 0) {  // the plan's state block, zeros
     d_state_ = reinterpret_cast<int32_t*>(d + state_off);
@@ -134,14 +175,14 @@ void Plan::upload_tables(int l0, int l1, std::vector<char>& staging, int64_t sta
     LevelPlan& L = levels_[l];
     const size_t n = L.segs.size();
     size_t o = offs[l - l0];
-    std::memcpy(staging.data() + o, L.segs.data(), n * sizeof(SegDesc));
+    std::memcpy(staging.p + o, L.segs.data(), n * sizeof(SegDesc));
     L.d_segs = reinterpret_cast<SegDesc*>(d + o);
     o = align(o + n * sizeof(SegDesc));
-    std::memcpy(staging.data() + o, L.vbase.data(), (n + 1) * sizeof(int64_t));
+    std::memcpy(staging.p + o, L.vbase.data(), (n + 1) * sizeof(int64_t));
     L.d_vbase = reinterpret_cast<int64_t*>(d + o);
-    std::memcpy(staging.data() + o + (n + 1) * sizeof(int64_t), L.fbase.data(), (n + 1) * sizeof(int64_t));
+    std::memcpy(staging.p + o + (n + 1) * sizeof(int64_t), L.fbase.data(), (n + 1) * sizeof(int64_t));
     L.d_fbase = reinterpret_cast<int64_t*>(d + o) + (n + 1);
-    std::memcpy(staging.data() + o + 2 * (n + 1) * sizeof(int64_t), L.cbase.data(), (n + 1) * sizeof(int64_t));
+    std::memcpy(staging.p + o + 2 * (n + 1) * sizeof(int64_t), L.cbase.data(), (n + 1) * sizeof(int64_t));
     L.d_cbase = reinterpret_cast<int64_t*>(d + o) + 2 * (n + 1);
     if (!L.d_mflags) {
       if (l == 0 && d_state_)  // the first level's mesh flags live in the state block
@@ -150,7 +191,7 @@ void Plan::upload_tables(int l0, int l1, std::vector<char>& staging, int64_t sta
         L.d_mflags = arena_.get_n<int>((int64_t)std::max<size_t>(n, 1));
     }
   }
-  SLAK_HIP_CHECK(hipMemcpyAsync(d, staging.data(), total, hipMemcpyHostToDevice, st_));
+  SLAK_HIP_CHECK(hipMemcpyAsync(d, staging.p, total, hipMemcpyHostToDevice, st_));
 }
 
 std::string Plan::describe_failure(int fl) const {

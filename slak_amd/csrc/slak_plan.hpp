@@ -117,7 +117,13 @@ class Plan {
   void record_analysis(LevelPlan& L);
   void record_level(int l);
   // state_ints > 0: also carries the plan's zeroed state block (first upload)
-  void upload_tables(int l0, int l1, std::vector<char>& staging, int64_t state_ints = 0);
+  // pinned host staging of a table upload (recycled through a process-wide
+  // pool; released once the plan's stream has passed the copy)
+  struct Staging {
+    char* p = nullptr;
+    size_t cap = 0;
+  };
+  void upload_tables(int l0, int l1, Staging& staging, int64_t state_ints = 0);
   int readback(const int32_t* src, int count, int64_t* dst);
 
   int scheme_;
@@ -134,7 +140,7 @@ class Plan {
   int32_t* d_state_ = nullptr;
   int64_t n_state_ = 0;
   int32_t* d_cnt_ = nullptr;  // first level: valence counters (zeroed by the gather)
-  std::vector<char> staging_first_, staging_all_;  // segment table uploads
+  Staging staging_first_, staging_all_;  // segment table uploads
   // first-level union gather (multi-mesh)
   float4* P1_ = nullptr;
   bool prepared_ = false;
