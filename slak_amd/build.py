@@ -68,6 +68,14 @@ def build(verbose: bool = True) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"drop-in test build failed:\n{r.stderr}")
+    # GPU::CSCMesh / OBJ headers used by a C++ program
+    mesh_test = os.path.join(LIBDIR, "slak_mesh_test")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "mesh_test.cpp"), "-L", LIBDIR, "-lslak",
+           "-Wl,-rpath,$ORIGIN", "-o", mesh_test]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"mesh test build failed:\n{r.stderr}")
     # command-line driver with the reference demo's interface
     cli = os.path.join(LIBDIR, "slak_cli")
     cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),

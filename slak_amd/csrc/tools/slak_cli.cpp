@@ -1,15 +1,12 @@
 // Command-line driver with the reference demo's interface (source/main.cpp:33-119):
 //   slak_cli <mesh.obj> <scheme 0=LAK,1=SLAK> <level> <save 0|1>
 //   slak_cli <config file>        (lines "<mesh.obj> <scheme> <level> <save>", '#' comments)
-// Reads the OBJ like include/obj.h:107-254 (1-based and negative indices,
-// v/vt/vn face forms, TRI/QUAD/POLY detection, uniform face order < 3
-// rejected), builds the device mesh like GPU::CSCMesh (include/Mesh.h:153-178,
-// 289-300: vals = 1-based corner position, w = 1, nverts = max index + 1),
-// refines through the drop-in headers and writes
-// <prefix>_<level>_<LAK|SLAK>.obj like GPU::CSCMesh::toObj / OBJ::write
-// (include/Mesh.h:379-412, include/obj.h:260-297: default stream precision).
-// With --parse-only it only reads the mesh and prints its sizes (no GPU).
-#include <algorithm>
+// the same steps as the demo: GPU::CSCMesh from the OBJ (include/obj.h,
+// include/Mesh.h of this repository), the conversion to MeshInfo, the drop-in
+// subdivideIteratively and CSCMesh::toObj to <prefix>_<level>_<LAK|SLAK>.obj.
+// Differences: an error is printed and the exit status is 1 (the reference
+// swallows it and stops), blank or malformed config lines are skipped.
+// With --parse-only it only runs OBJ::read and prints what it read (no GPU).
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -21,143 +18,31 @@
 
 #include "CatmullClark.h"
 #include "LAKCatmullClark.h"
-
-namespace {
-
-struct Obj {
-  int type = SLAK_POLY;
-  int max_face_order = 0;
-  std::vector<int> f_offs{0};
-  std::vector<int> f_vi;
-  std::vector<float> v;  // 3 per vertex
-};
-
-Obj read_obj(const std::string& path) {
-  std::ifstream in(path);
-  if (!in.is_open()) throw std::runtime_error("Could not open obj file\n");
-  Obj o;
-  std::string line;
-  while (std::getline(in, line)) {
-    if (line.empty()) continue;
-    std::stringstream ss(line);
-    std::string id;
-    ss >> id;
-    if (id == "v") {
-      float x = 0, y = 0, z = 0;
-      ss >> x >> y >> z;
-      o.v.insert(o.v.end(), {x, y, z});
-    } else if (id == "f") {
-      int n = 0, vi = -1;
-      const int nv = (int)(o.v.size() / 3);
-      while (ss >> vi && !ss.fail()) {
-        o.f_vi.push_back(vi < 0 ? vi + nv : vi - 1);
-        if (ss.peek() == '/') {  // skip texture / normal indices
-          ss.get();
-          int t = 0;
-          if (!(ss >> t)) ss.clear();
-          if (ss.peek() == '/') {
-            ss.get();
-            int nn = 0;
-            if (!(ss >> nn)) ss.clear();
-          }
-        }
-        vi = -1;
-        ++n;
-      }
-      o.f_offs.push_back(o.f_offs.back() + n);
-    }
-  }
-  int lo = 1 << 30;
-  for (size_t f = 1; f < o.f_offs.size(); ++f) {
-    int order = o.f_offs[f] - o.f_offs[f - 1];
-    lo = std::min(lo, order);
-    o.max_face_order = std::max(o.max_face_order, order);
-  }
-  if (o.f_offs.size() <= 1 || lo != o.max_face_order || o.max_face_order > 4)
-    o.type = SLAK_POLY;
-  else if (o.max_face_order == 4)
-    o.type = SLAK_QUAD;
-  else if (o.max_face_order == 3)
-    o.type = SLAK_TRI;
-  else
-    throw std::runtime_error("Error loading obj: Face with less than three vertices detected.\n");
-  return o;
-}
-
-template <class T>
-T* upload(const std::vector<T>& h) {
-  void* p = nullptr;
-  slak_dropin::succeed(slak_malloc(&p, std::max<size_t>(1, h.size()) * sizeof(T)));
-  if (!h.empty()) slak_dropin::succeed(slak_memcpy_h2d(p, h.data(), h.size() * sizeof(T)));
-  return static_cast<T*>(p);
-}
-
-template <class T>
-std::vector<T> download(const T* d, size_t n) {
-  std::vector<T> h(n);
-  if (n) slak_dropin::succeed(slak_memcpy_d2h(h.data(), d, n * sizeof(T)));
-  return h;
-}
-
-using LMesh = LAKCatmullClark::MeshInfo<int, int, int, float>;
-
-LMesh to_device(const Obj& o) {
-  LMesh m;
-  m.type = static_cast<LMesh::MeshType>(o.type);
-  m.nfaces = o.f_offs.size() - 1;
-  m.nnz = o.f_vi.size();
-  m.max_face_order = (size_t)o.max_face_order;
-  m.is_reduced = true;
-  // GPU::CSCMesh::setFromCPU sizes the vertex array by the largest referenced index
-  int vmax = -1;
-  for (int id : o.f_vi) vmax = std::max(vmax, id);
-  m.nverts = (size_t)(vmax + 1);
-  std::vector<int> vals(o.f_vi.size());
-  for (size_t f = 0; f + 1 < o.f_offs.size(); ++f)
-    for (int k = o.f_offs[f]; k < o.f_offs[f + 1]; ++k) vals[k] = k - o.f_offs[f] + 1;
-  std::vector<float> v4(4 * m.nverts, 1.0f);
-  for (size_t i = 0; i < m.nverts && 3 * i + 2 < o.v.size(); ++i)
-    for (int c = 0; c < 3; ++c) v4[4 * i + c] = o.v[3 * i + c];
-  m.ptr = upload(o.f_offs);
-  m.ids = upload(o.f_vi);
-  m.vals = upload(vals);
-  m.verts = upload(v4);
-  return m;
-}
-
-void write_obj(const LMesh& r, const std::string& fname) {
-  std::ofstream out(fname);
-  if (!out.is_open()) {
-    std::cerr << "Error opening mesh file for writing!" << std::endl;
-    return;
-  }
-  auto verts = download(r.verts, 4 * r.nverts);
-  auto ids = download(r.ids, r.nnz);
-  std::cout << "Writing file " << fname << std::endl;
-  std::cout << "Writing " << r.nverts << " vertices" << std::endl;
-  for (size_t i = 0; i < r.nverts; ++i)
-    out << "v " << verts[4 * i] << " " << verts[4 * i + 1] << " " << verts[4 * i + 2] << "\n";
-  // refined meshes are quads; LAK form is sorted by vals first (already in order)
-  std::cout << "Writing " << r.nnz / 4 << " faces" << std::endl;
-  for (size_t f = 0; f < r.nnz; f += 4)
-    out << "f " << ids[f] + 1 << " " << ids[f + 1] + 1 << " " << ids[f + 2] + 1 << " " << ids[f + 3] + 1 << "\n";
-  std::cout << "Done writing" << std::endl;
-}
-
-}  // namespace
+#include "Mesh.h"
 
 int main(int argc, char* argv[]) {
   std::vector<std::string> paths;
   std::vector<int> schemes, levels;
   std::vector<bool> saves;
   bool parse_only = argc >= 2 && std::string(argv[1]) == "--parse-only";
-  if (parse_only) {
+  if (parse_only) {  // OBJ::read only (no GPU): sizes, type and the other elements read
     for (int i = 2; i < argc; ++i) {
-      Obj o = read_obj(argv[i]);
+      OBJ::Data o;
+      OBJ::read(o, argv[i]);
       int vmax = -1;
       for (int id : o.f_vi) vmax = std::max(vmax, id);
-      std::printf("%s type=%d nf=%zu nnz=%zu nv=%d max_face_order=%d\n", argv[i], o.type, o.f_offs.size() - 1,
-                  o.f_vi.size(), vmax + 1, o.max_face_order);
+      std::printf("%s type=%d nf=%zu nnz=%zu nv=%d max_face_order=%d v=%zu vt=%zu vn=%zu f_vti=%zu f_vni=%zu tags=%zu\n",
+                  argv[i], (int)o.type, o.f_offs.size() - 1, o.f_vi.size(), vmax + 1, o.max_face_order, o.v.size(),
+                  o.vt.size(), o.vn.size(), o.f_vti.size(), o.f_vni.size(), o.t.size());
+      for (const auto& n : o.vn) std::printf("vn %.9g %.9g %.9g\n", n.x, n.y, n.z);
+      for (int x : o.f_vti) std::printf("vti %d\n", x);
+      for (int x : o.f_vni) std::printf("vni %d\n", x);
+      for (const auto& t : o.t) {
+        std::printf("tag %s", t.type.c_str());
+        for (int x : t.intargs) std::printf(" i%d", x);
+        for (float x : t.floatargs) std::printf(" f%.9g", x);
+        std::printf("\n");
+      }
     }
     return 0;
   }
@@ -191,18 +76,27 @@ int main(int argc, char* argv[]) {
   for (size_t i = 0; i < paths.size(); ++i) {
     try {
       std::cout << "Mesh: " << paths[i] << std::endl;
-      Obj o = read_obj(paths[i]);
-      LMesh cmesh = to_device(o), rmesh;
-      if (schemes[i] == 0)
-        LAKCatmullClark::subdivideIteratively(cmesh, rmesh, levels[i]);
-      else
-        CatmullClark::subdivideIteratively(cmesh, rmesh, levels[i]);
-      if (saves[i] && rmesh.ids) {
-        std::string prefix = paths[i].substr(0, paths[i].find_last_of('.'));
-        write_obj(rmesh, prefix + "_" + std::to_string(levels[i]) + "_" + (schemes[i] == 0 ? "LAK" : "SLAK") + ".obj");
+      const GPU::CSCMesh<int, int, int, float> cmesh(paths[i]);
+      GPU::CSCMesh<int, int, int, float> rmesh;
+      if (schemes[i] == 0) {
+        using MeshInfo = LAKCatmullClark::MeshInfo<int, int, int, float>;
+        MeshInfo cmesh_info, rmesh_info;
+        GPU::CSCMeshCCMeshInfoConversion(cmesh, cmesh_info);
+        LAKCatmullClark::subdivideIteratively(cmesh_info, rmesh_info, levels[i]);
+        GPU::CSCMeshCCMeshInfoConversion(rmesh_info, rmesh);  // rmesh owns the result
+      } else if (schemes[i] == 1) {
+        using MeshInfo = CatmullClark::MeshInfo<int, int, int, float>;
+        MeshInfo cmesh_info, rmesh_info;
+        GPU::CSCMeshCCMeshInfoConversion(cmesh, cmesh_info);
+        CatmullClark::subdivideIteratively(cmesh_info, rmesh_info, levels[i]);
+        GPU::CSCMeshCCMeshInfoConversion(rmesh_info, rmesh);
+      } else {
+        throw std::runtime_error("scheme must be 0 (LAK) or 1 (SLAK)");
       }
-      rmesh.freeAndReset();
-      cmesh.freeAndReset();
+      if (saves[i]) {
+        const std::string prefix = paths[i].substr(0, paths[i].find_last_of('.'));
+        rmesh.toObj(prefix + "_" + std::to_string(levels[i]) + "_" + (schemes[i] == 0 ? "LAK" : "SLAK") + ".obj");
+      }
     } catch (const char* msg) {  // the reference swallows errors; report them
       std::cerr << "error: " << msg << std::endl;
       status = 1;
