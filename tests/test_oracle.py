@@ -225,3 +225,22 @@ def test_fp32_cpu_baseline_topology_bit_exact(oracle):
     np.testing.assert_array_equal(a.ids, b.ids)
     diag = m.bbox_diag()
     assert np.abs(a.P - b.P).max() / diag < 1e-6
+
+
+@pytest.mark.parametrize("make", [g.single_quad, lambda: g.open_grid(5, 3, sigma=0.02, seed=3),
+                                  lambda: g.cylinder(9, 4)], ids=["quad", "open_grid", "cylinder"])
+@pytest.mark.parametrize("scheme", [0, 1], ids=["LAK", "SLAK"])
+def test_boundary_loops_are_bspline_curves(oracle, make, scheme):
+    """The boundary rule (3/4 P + 1/8 neighbours, midpoints, no corner rule)
+    is cubic B-spline curve subdivision along each boundary loop: a textbook
+    closed form that pins the oracle's boundary path."""
+    from tests.test_gpu_boundary import boundary_loops, bspline_closed
+    m = make()
+    loops0 = boundary_loops(m.face_ptr(), m.ids)
+    for L in (1, 2, 3):
+        r = oracle.subdivide(m, L, scheme=scheme, precision=64)
+        by_start = {lp[0]: lp for lp in boundary_loops(np.arange(r.nf + 1) * 4, r.ids)}
+        for lp0 in loops0:
+            lp = by_start[lp0[0]]
+            ref = bspline_closed(m.verts[lp0, :3].astype(np.float64), L)
+            assert np.abs(r.P[lp, :3] - ref).max() <= 1e-12
