@@ -53,8 +53,8 @@ void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int3
 void launch_transpose(bool quad, const int32_t* ids, const int32_t* fptr, int32_t nf, const int32_t* tptr,
                       int32_t* cnt, int32_t* tids, const int* guard, hipStream_t st);
 void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
-// first level: edge numbering and face points in one launch
-void launch_number_faces(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
+// first level: edge numbering and (faces set) face points in one launch
+void launch_number_faces(bool quad, int scheme, int cap, const LevelView& L, bool faces, hipStream_t st);
 void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st);
 // refine (per corner) and vertex (per vertex) items of a level in one launch
 void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st);

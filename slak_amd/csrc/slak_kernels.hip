@@ -1398,9 +1398,9 @@ void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStre
   SLAK_ROW_DISPATCH(k_symbolic, quad, scheme, cap, g, L, st);
 }
 
-void launch_number_faces(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st) {
+void launch_number_faces(bool quad, int scheme, int cap, const LevelView& L, bool faces, hipStream_t st) {
   const int32_t nbn = L.nv > 0 ? (int32_t)grid_for(L.nv) : 0;
-  const int32_t nbf = L.nf > 0 ? (int32_t)grid_for(L.nf) : 0;
+  const int32_t nbf = faces && L.nf > 0 ? (int32_t)grid_for(L.nf) : 0;
   if (nbn + nbf == 0) return;
   const unsigned g = (unsigned)(nbn + nbf);
   if (quad) {

@@ -134,12 +134,26 @@ Plan::~Plan() {
 
 // Reads the flags word and the count ints after it in the state block in one
 // copy (count = 0: flags only) and waits for it.
-int Plan::readback(const int32_t* src, int count, int64_t* dst) {
+int Plan::readback(const int32_t* src, int count, int64_t* dst, const std::function<void()>& overlap) {
   (void)src;
   int64_t* pin = pinned_buffer((size_t)count / 2 + 2);
   int32_t* p32 = reinterpret_cast<int32_t*>(pin);
   SLAK_HIP_CHECK(hipMemcpyAsync(p32, d_flags_, (1 + count) * sizeof(int32_t), hipMemcpyDeviceToHost, st_));
-  SLAK_HIP_CHECK(hipStreamSynchronize(st_));
+  if (overlap) {
+    struct Ev {
+      hipEvent_t e = nullptr;
+      ~Ev() {
+        if (e) (void)hipEventDestroy(e);
+      }
+    };
+    thread_local Ev ev;
+    if (!ev.e) SLAK_HIP_CHECK(hipEventCreateWithFlags(&ev.e, hipEventDisableTiming));
+    SLAK_HIP_CHECK(hipEventRecord(ev.e, st_));
+    overlap();
+    SLAK_HIP_CHECK(hipEventSynchronize(ev.e));
+  } else {
+    SLAK_HIP_CHECK(hipStreamSynchronize(st_));
+  }
   ++syncs_;
   for (int i = 0; i < count; ++i) dst[i] = p32[1 + i];
   return p32[0];
@@ -353,6 +367,13 @@ void Plan::prepare() {
   plan_first_level();
   LevelPlan& L1 = levels_[0];
   const int nseg = (int)L1.segs.size();
+  // a single mesh numbers its edges before the read-back: the first upload
+  // gives it nnz >= ne as the edge-id bound and a flag buffer of that size
+  const bool early = nseg == 1 && jobs_.size() == 1 && jobs_[L1.jobs[0]].known_ne < 0;
+  if (early) {
+    L1.segs[0].ne = L1.segs[0].nnz;
+    L1.bnd = arena_.get_n<int32_t>(L1.nnz);
+  }
   n_state_ = 1 + 2 * kRowHistBins + (nseg + 1) + nseg;
   upload_tables(0, 1, staging_first_, n_state_);
   int64_t need = scan_partials_len(L1.nv);
@@ -388,7 +409,10 @@ void Plan::prepare() {
   std::vector<int64_t> hb(2 * kRowHistBins + nseg + 1, 0);
   bool have_hist = false;
   if (!all_known) {
-    fail_if(readback(nullptr, 2 * kRowHistBins + nseg + 1, hb.data()));
+    std::function<void()> overlap;
+    if (early) overlap = [this, &L1] { launch_number(L1, false); };
+    fail_if(readback(nullptr, 2 * kRowHistBins + nseg + 1, hb.data(), overlap));
+    number_early_ = early;
     have_hist = true;
   } else {
     if (!all_trusted) fail_if(readback(nullptr, 0, nullptr));
@@ -636,7 +660,7 @@ void Plan::prepare() {
     for (int l = 1; l <= Lstruct; ++l) {
       LevelPlan& L = levels_[l - 1];
       if (!L.boundary) continue;
-      L.bnd = bnd;
+      if (!(l == 1 && number_early_)) L.bnd = bnd;  // the early numbering wrote level 1's own
       L.blk = blk;
       if (l >= 2) levels_[l - 2].nbnd = bnd;
     }
@@ -660,32 +684,45 @@ void Plan::prepare() {
   prepared_ = true;
 }
 
+// The first level's edge numbering (E of the mapped SpGEMM: edge id and twin
+// of every half-edge, boundary flags), with its face points in the same
+// launch when faces is set.
+void Plan::launch_number(LevelPlan& L, bool faces) {
+  const double poly_b = L.quad ? 0.0 : 4.0 * (L.nf + 1) + 4.0 * L.nnz;
+  const double ne = (double)L.ne;
+  LevelView V{};
+  V.ids = L.ids;
+  V.fptr = L.fptr;
+  V.cface = L.cface;
+  V.P = L.P;
+  V.nv = (int32_t)L.nv;
+  V.nf = (int32_t)L.nf;
+  V.nnz = L.nnz;
+  V.tptr = L.tptr;
+  V.tids = L.tids;
+  V.eptr = L.eptr;
+  V.ci = reinterpret_cast<int32_t*>(L.ci);
+  V.bnd = L.bnd;
+  V.flags = d_flags_;
+  V.seg = L.table();
+  ProfScope ps(st_, faces ? (L.quad ? "number_facepoints_quad" : "number_facepoints_poly") : "number",
+               12.0 * L.nv + 12.0 * L.nnz + 8.0 * L.nnz + (L.bnd ? 4.0 * ne : 0.0) + poly_b +
+                   (faces ? 4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf : 0.0));
+  launch_number_faces(L.quad, scheme_, L.cap, V, faces, st_);
+}
+
 void Plan::record_level(int l) {
   LevelPlan& L = levels_[l - 1];
   const double poly_b = L.quad ? 0.0 : 4.0 * (L.nf + 1) + 4.0 * L.nnz;
   const double ne = (double)L.ne;
-  if (l == 1) {
+  if (l == 1 && !number_early_) {
     // edge numbering of the first level (later levels get theirs in closed form)
-    LevelView V{};
-    V.ids = L.ids;
-    V.fptr = L.fptr;
-    V.cface = L.cface;
-    V.P = L.P;
-    V.nv = (int32_t)L.nv;
-    V.nf = (int32_t)L.nf;
-    V.nnz = L.nnz;
-    V.tptr = L.tptr;
-    V.tids = L.tids;
-    V.eptr = L.eptr;
-    V.ci = reinterpret_cast<int32_t*>(L.ci);
-    V.bnd = L.bnd;
-    V.flags = d_flags_;
-    V.seg = L.table();
-    // with the first level's face points in the same launch
-    ProfScope ps(st_, L.quad ? "number_facepoints_quad" : "number_facepoints_poly",
-                 12.0 * L.nv + 12.0 * L.nnz + 8.0 * L.nnz + (L.bnd ? 4.0 * ne : 0.0) + poly_b + 4.0 * L.nnz +
-                     16.0 * L.nv + 16.0 * L.nf);
-    launch_number_faces(L.quad, scheme_, L.cap, V, st_);
+    launch_number(L, true);
+  }
+  if (l == 1 && number_early_) {
+    ProfScope ps(st_, L.quad ? "facepoints_quad" : "facepoints_poly",
+                 4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + (L.quad ? 0.0 : 4.0 * (L.nf + 1)));
+    launch_faces(L.quad, true, false, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), nullptr, nullptr, st_);
   }
   if (L.blk) {
     ProfScope ps(st_, "scan", 8.0 * ne + 4);
@@ -787,6 +824,7 @@ void Plan::launch_lak_forms() {
 
 void Plan::launch(bool include_analysis) {
   if (levels_.empty()) return;
+  if (include_analysis) number_early_ = false;  // the analysis runs again: number in order
   LevelPlan& L1 = levels_[0];
   if (!lak_forms_in_place_) launch_lak_forms();
   if (include_analysis) {

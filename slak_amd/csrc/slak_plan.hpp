@@ -5,6 +5,8 @@
 // one hipGraph.
 #pragma once
 
+#include <functional>
+
 #include <hip/hip_runtime.h>
 
 #include <memory>
@@ -124,7 +126,10 @@ class Plan {
     size_t cap = 0;
   };
   void upload_tables(int l0, int l1, Staging& staging, int64_t state_ints = 0);
-  int readback(const int32_t* src, int count, int64_t* dst);
+  // overlap: work enqueued after the copy and before the wait (the host then
+  // waits for the copy only)
+  int readback(const int32_t* src, int count, int64_t* dst, const std::function<void()>& overlap = {});
+  void launch_number(LevelPlan& L, bool faces);
 
   int scheme_;
   std::vector<MeshJob> jobs_;
@@ -157,6 +162,9 @@ class Plan {
   // the last level is computed from that mesh's structure (k_final) and the
   // level before it writes no structure
   bool fused_final_ = false;
+  // single mesh: the first level's edge numbering ran during the analysis
+  // read-back (the level then launches its face points only)
+  bool number_early_ = false;
   void launch_lak_forms();
   std::vector<void*> pooled_outputs_;
 };
