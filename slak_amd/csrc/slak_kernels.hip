@@ -482,7 +482,7 @@ __device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K,
   const bool final_level = S->dst_final != 0;  // the caller's output: not read again here
   if (final_level) {
     store_nt(S->dst_ids + 4 * lt, c.v - K.vb + K.off, epb + c.e1, fpid, epb + c.e0);
-  } else {
+  } else if (S->dst_ids) {  // null: the next level is computed without them (k_final)
     reinterpret_cast<int4*>(S->dst_ids)[lt] = make_int4(c.v - K.vb + K.off, epb + c.e1, fpid, epb + c.e0);
   }
   if (c.own) {
@@ -770,15 +770,17 @@ __global__ void __launch_bounds__(kBlock) k_level(RefineView R, int32_t nbr, int
 
 // ---------------------------------------------------------------------------
 // Final level of a single mesh in closed form from the quad mesh M two levels
-// before the output. M1 (the refinement of M, whose positions P1 and face
-// points are in hand) has corners 4t + k for the corners t of M:
+// before the output. M1 (the refinement of M, whose positions P1 are in hand;
+// its ids and structure are never written) has corners 4t + k for the
+// corners t of M:
 //   quad t of M1   (v, ep(e1), fp(F), ep(e0))  in M1 vertex ids
 //   M1 edge ids    corner 4t + k leaves along E1[k] = (ex, ey, ez, ew) and
 //                  enters along E0[k] = (ew, ex, ey, ez), the ids the refine
 //                  items give M1 (LAK: 4e - bl(e) + rank, SLAK: M^T positions)
 //   output quad    of M1 corner 4t + k: (vertex, ep'(E1[k]), fp'(t), ep'(E0[k]))
-// One lane per corner t of M computes its four output quads (64 contiguous
-// bytes, written out by the block through LDS). The owner of edge e1 of M (t < twin, or no twin) writes the edge
+// One lane per corner t of M computes the face point of M1 face t from its
+// four corners (the fourth comes from the previous lane) and its four output
+// quads (64 contiguous bytes, written out by the block through LDS). The owner of edge e1 of M (t < twin, or no twin) writes the edge
 // points of the three or four M1 edges around ep(e1) (both halves of e1 and
 // the edges to fp(F) and fp(G)), contiguous for LAK, and the vertex point of
 // ep(e1), from the same nine positions. Lane 0 of each quad writes the vertex
@@ -802,7 +804,19 @@ __device__ __forceinline__ float4 quad_next(float4 x) {
                      __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.z), ctrl, 0xF, 0xF, false)),
                      __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.w), ctrl, 0xF, 0xF, false)));
 }
+__device__ __forceinline__ float4 quad_prev(float4 x) {
+  constexpr int ctrl = 0x93;  // [3, 0, 1, 2]
+  return make_float4(__int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.x), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.y), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.z), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.w), ctrl, 0xF, 0xF, false)));
+}
 __device__ __forceinline__ float4 f4sel(bool c, float4 a, float4 b) { return c ? a : b; }
+// face point of a quad from its corners' positions in corner order (the sum
+// and division k_faces uses)
+__device__ __forceinline__ float4 face_point(float4 a, float4 b, float4 c, float4 d) {
+  return f4div(f4add(f4add(f4add(a, b), c), d), 4.0f);
+}
 
 constexpr int kStageStride = 5;  // int4 per lane in the staging area (4 used; 5 spreads the banks)
 
@@ -826,7 +840,6 @@ __device__ __forceinline__ void final_corner(const FinalView& V, int32_t t, int4
   const int32_t epb2 = nv1 + nnz;    // output vertex of the edge point of M1 edge 0
   const float4* __restrict__ P1 = V.P1;
   float4* __restrict__ out = V.out_verts;
-  const float4* __restrict__ fpp = out + nv1;  // face points of the M1 faces (= corners of M)
   int32_t bl1 = 0, bl0 = 0;
   if (BND) {
     bl1 = V.blk[e1];
@@ -838,20 +851,33 @@ __device__ __forceinline__ void final_corner(const FinalView& V, int32_t t, int4
     if (s0 >= 0) mp_s0 = V.mpos[s0];
     if (own && s1 >= 0) mp_s1 = V.mpos[s1];
   }
+  int2 cs0 = make_int2(0, 0), cn1 = make_int2(0, 0);  // (edge, twin) of prev(s1) and next(s1)
+  if (own && s1 >= 0) {
+    cs0 = V.ci[(s1 & ~3) | ((s1 + 3) & 3)];
+    cn1 = V.ci[n1];
+  }
   const float4 Pe = P1[ep0 + e1];   // ep(e1)
   const float4 Pv = P1[v];
   const float4 PF = P1[nv + F];     // fp(F)
-  const float4 fpt = fpp[t];
+  // from the other lanes of the quad (all four run this): w is the vertex of
+  // the next corner, ep(e0) the edge point of the previous one
+  const float4 Pw = quad_next(Pv), Pu = quad_prev(Pe);
+  // face point of M1 face t = (v, ep(e1), fp(F), ep(e0)), as k_faces forms it;
+  // it is an output position too
+  const float4 fpt = face_point(Pv, Pe, PF, Pu);
+  out[nv1 + t] = fpt;
+  const float4 fptn = quad_next(fpt);
+  const float4 fq1 = quad_lane<1>(fpt), fq2 = quad_lane<2>(fpt), fq3 = quad_lane<3>(fpt);
+  const float4 pq1 = quad_lane<1>(Pe), pq2 = quad_lane<2>(Pe), pq3 = quad_lane<3>(Pe);
+  // across e1: M1 faces s1 = (w, ep(e1), fp(G), ep(e0(s1))) and
+  // next(s1) = (v, ep(e1(next(s1))), fp(G), ep(e1))
   float4 PG = make_float4(0.f, 0.f, 0.f, 0.f), fps = PG, fpsn = PG;
   if (own && s1 >= 0) {
     PG = P1[nv + g1];
-    fps = fpp[s1];
-    fpsn = fpp[n1];
+    const float4 Pa = P1[ep0 + clampi(cs0.x, emax)], Pb = P1[ep0 + clampi(cn1.x, emax)];
+    fps = face_point(Pw, Pe, PG, Pa);
+    fpsn = face_point(Pv, Pb, PG, Pe);
   }
-  // from the other lanes of the quad (all four run this)
-  const float4 Pw = quad_next(Pv), fptn = quad_next(fpt);
-  const float4 fq1 = quad_lane<1>(fpt), fq2 = quad_lane<2>(fpt), fq3 = quad_lane<3>(fpt);
-  const float4 pq1 = quad_lane<1>(Pe), pq2 = quad_lane<2>(Pe), pq3 = quad_lane<3>(Pe);
 
   // M1 edge ids of the corners 4t .. 4t+3 (as the refine items number M1)
   int32_t ex, ey, ez, ew;
@@ -960,28 +986,36 @@ __device__ __forceinline__ void final_corner(const FinalView& V, int32_t t, int4
 template <int SCHEME, bool BND>
 __device__ __forceinline__ void final_vertex(const FinalView& V, int32_t v) {
   if (v >= V.nv) return;
-  const int32_t ep0 = V.nv + V.nf, nv1 = ep0 + V.ne;
+  const int32_t ep0 = V.nv + V.nf;
   const int32_t emax = V.ne > 0 ? V.ne - 1 : 0;
   const float4* __restrict__ P1 = V.P1;
-  const float4* __restrict__ fpp = V.out_verts + nv1;
   const int2* __restrict__ ci = V.ci;
   const int32_t p0 = V.tptr[v];
   const int n = V.tptr[v + 1] - p0;
   const int2* __restrict__ row = V.trow + p0;
   const float4 Pv = P1[v];
   auto prevc = [](int32_t t) __attribute__((always_inline)) { return (t & ~3) | ((t + 3) & 3); };
+  // face point of M1 face t = (v, ep(e1(t)), fp(face of t), ep(e0(t)))
+  auto fp_of = [&](int32_t t, int2 c1, float4 pe0) __attribute__((always_inline)) {
+    return face_point(Pv, P1[ep0 + clampi(c1.x, emax)], P1[V.nv + (t >> 2)], pe0);
+  };
   int32_t tt[kVertexRegs];
   int2 cp[kVertexRegs], co[kVertexRegs];
-  float4 a[kVertexRegs];
+  float4 a[kVertexRegs], b[kVertexRegs];
 #pragma unroll
   for (int i = 0; i < kVertexRegs; ++i)
     if (i < n) tt[i] = row[i].x;
 #pragma unroll
   for (int i = 0; i < kVertexRegs; ++i)
     if (i < n) {
-      a[i] = fpp[tt[i]];
       cp[i] = ci[prevc(tt[i])];
-      if (BND) co[i] = ci[tt[i]];
+      co[i] = ci[tt[i]];
+    }
+#pragma unroll
+  for (int i = 0; i < kVertexRegs; ++i)
+    if (i < n) {
+      b[i] = P1[ep0 + clampi(cp[i].x, emax)];
+      a[i] = fp_of(tt[i], co[i], b[i]);
     }
   bool boundary = false;
   if (BND) {
@@ -995,15 +1029,14 @@ __device__ __forceinline__ void final_vertex(const FinalView& V, int32_t v) {
   }
   float4 q;
   if (!boundary) {
-    float4 b[kVertexRegs];
-#pragma unroll
-    for (int i = 0; i < kVertexRegs; ++i)
-      if (i < n) b[i] = P1[ep0 + clampi(cp[i].x, emax)];
     float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int i = 0; i < kVertexRegs; ++i)
       if (i < n) s = f4add(s, a[i]);
-    for (int i = kVertexRegs; i < n; ++i) s = f4add(s, fpp[row[i].x]);
+    for (int i = kVertexRegs; i < n; ++i) {
+      const int32_t t = row[i].x;
+      s = f4add(s, fp_of(t, ci[t], P1[ep0 + clampi(ci[prevc(t)].x, emax)]));
+    }
 #pragma unroll
     for (int i = 0; i < kVertexRegs; ++i)
       if (i < n) s = f4add(s, b[i]);

@@ -513,6 +513,10 @@ void Plan::prepare() {
     op += npo;
   }
 
+  // single mesh, two or more levels, quad mesh two levels before the output:
+  // k_final computes the last level (see fused_final_)
+  fused_final_ = jobs_.size() == 1 && Lmax >= 2 && (Lmax >= 3 || levels_[0].quad);
+  if (const char* e = std::getenv("SLAK_FUSED_FINAL")) fused_final_ &= std::atoi(e) != 0;
   // level l (1-based) consumes sizes index l-1 and produces index l
   int64_t max_nv = 0, max_nnz = 0;
   levels_.resize(Lmax);
@@ -567,8 +571,10 @@ void Plan::prepare() {
         nv_next += jobs_[j].nv[l];
         nnz_next += jobs_[j].nnz[l];
       }
+    // the level before a fused last level hands on positions only
+    const bool ids_next = !(fused_final_ && l == Lmax - 1);
     float4* Un_P = nv_next ? arena_.get_n<float4>(nv_next) : nullptr;
-    int32_t* Un_ids = nnz_next ? arena_.get_n<int32_t>(nnz_next) : nullptr;
+    int32_t* Un_ids = nnz_next && ids_next ? arena_.get_n<int32_t>(nnz_next) : nullptr;
     int64_t vbn = 0, cbn = 0, ebn = 0;
     for (size_t k = 0; k < L.segs.size(); ++k) {
       MeshJob& J = jobs_[L.jobs[k]];
@@ -589,7 +595,7 @@ void Plan::prepare() {
         S.dst_final = 1;
       } else {
         S.dst_verts = Un_P + vbn;
-        S.dst_ids = Un_ids + cbn;
+        S.dst_ids = Un_ids ? Un_ids + cbn : nullptr;
         S.dst_ptr = nullptr;
         S.dst_vals = nullptr;
         S.id_offset = vbn;
@@ -610,8 +616,6 @@ void Plan::prepare() {
       nx.P = Un_P;
     }
   }
-  fused_final_ = jobs_.size() == 1 && Lmax >= 2 && levels_[Lmax - 2].quad && levels_[Lmax - 1].segs.size() == 1;
-  if (const char* e = std::getenv("SLAK_FUSED_FINAL")) fused_final_ &= std::atoi(e) != 0;
   if (fused_final_)
     for (SegDesc& S : levels_[Lmax - 2].segs) {  // the level before the last writes no structure
       S.next_vb = -1;
@@ -728,12 +732,13 @@ void Plan::record_level(int l) {
     ProfScope ps(st_, "scan", 8.0 * ne + 4);
     launch_scan(L.bnd, L.blk, L.ne, d_partial_, d_flags_, st_);
   }
-  if (l > 1) {
+  const bool fused_last = fused_final_ && l == (int)levels_.size();  // k_final forms the face points
+  if (l > 1 && !fused_last) {
     ProfScope ps(st_, L.quad ? "facepoints_quad" : "facepoints_poly",
                  4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + (L.quad ? 0.0 : 4.0 * (L.nf + 1)));
     launch_faces(L.quad, true, false, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), nullptr, nullptr, st_);
   }
-  if (fused_final_ && l == (int)levels_.size()) {
+  if (fused_last) {
     const LevelPlan& M = levels_[l - 2];
     FinalView V{};
     V.ids = M.ids;
@@ -750,12 +755,12 @@ void Plan::record_level(int l) {
     V.out_verts = L.segs[0].dst_verts;
     V.out_ids = L.segs[0].dst_ids;
     // algorithmic bytes: reads the ids, (edge, twin) pairs, M^T rows and
-    // offsets of M (boundary scan, SLAK positions), the positions of M1 and
-    // its face points; writes the output quads, vertex points and edge points
+    // offsets of M (boundary scan, SLAK positions) and the positions of M1;
+    // writes the output quads and every output position
     const double mne = (double)M.ne;
     const double reads = 4.0 * M.nnz + 8.0 * M.nnz + 8.0 * M.nnz + 4.0 * M.nv + (M.blk ? 4.0 * mne : 0.0) +
-                         (scheme_ == kSLAK ? 4.0 * M.nnz : 0.0) + 16.0 * L.nv + 16.0 * L.nf;
-    const double writes = 16.0 * L.nnz + 16.0 * L.nv + 16.0 * ne;
+                         (scheme_ == kSLAK ? 4.0 * M.nnz : 0.0) + 16.0 * L.nv;
+    const double writes = 16.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 16.0 * ne;
     ProfScope ps(st_, "refine_vertex_final_quad", reads + writes);
     launch_final(scheme_, V, st_);
     return;
