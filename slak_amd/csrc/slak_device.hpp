@@ -461,6 +461,8 @@ struct FinalView {
   const float4* P1;     // positions of M1 = [M vertices | M face points | M edge points]
   float4* out_verts;    // output positions: [M1 vertices | M1 face points (already written) | M1 edge points]
   int32_t* out_ids;     // output quads: 4 per M1 corner, 16 per M corner
+  int32_t* out_ptr;     // LAK form of the output (ptr[i] = 4i, vals cyclic 1..4), or null
+  int32_t* out_vals;
 };
 
 // Face, successor and predecessor corner of corner t (quad: bit arithmetic;

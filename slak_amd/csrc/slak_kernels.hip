@@ -1082,6 +1082,24 @@ __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int3
         store_nt(dst + 4 * i, x.x, x.y, x.z, x.w);
       }
     }
+    // LAK form of the block's 4 x 256 output quads (createQuadColPtr /
+    // createQuadVals): every int4 of vals is (1, 2, 3, 4) and int4 q of ptr
+    // is 16q + (0, 4, 8, 12), so consecutive lanes take consecutive pieces
+    if (V.out_vals) {
+      int32_t* __restrict__ vals = V.out_vals + 16 * t0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = j * kBlock + threadIdx.x;
+        if (i < nq) store_nt(vals + 4 * i, 1, 2, 3, 4);
+      }
+    }
+    if (V.out_ptr) {
+      if (t < V.nnz) {
+        const int32_t q = (int32_t)t;
+        store_nt(V.out_ptr + 4 * (int64_t)q, 16 * q, 16 * q + 4, 16 * q + 8, 16 * q + 12);
+      }
+      if (t == (int64_t)V.nnz - 1) store_nt(V.out_ptr + 4 * (int64_t)V.nnz, 16 * V.nnz);
+    }
   }
 }
 

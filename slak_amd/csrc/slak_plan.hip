@@ -616,6 +616,7 @@ void Plan::prepare() {
       nx.P = Un_P;
     }
   }
+  if (fused_final_) levels_[Lmax - 1].lak_form = false;  // k_final writes it
   if (fused_final_)
     for (SegDesc& S : levels_[Lmax - 2].segs) {  // the level before the last writes no structure
       S.next_vb = -1;
@@ -754,13 +755,16 @@ void Plan::record_level(int l) {
     V.P1 = L.P;
     V.out_verts = L.segs[0].dst_verts;
     V.out_ids = L.segs[0].dst_ids;
+    V.out_ptr = L.segs[0].dst_ptr;
+    V.out_vals = L.segs[0].dst_vals;
     // algorithmic bytes: reads the ids, (edge, twin) pairs, M^T rows and
     // offsets of M (boundary scan, SLAK positions) and the positions of M1;
     // writes the output quads and every output position
     const double mne = (double)M.ne;
     const double reads = 4.0 * M.nnz + 8.0 * M.nnz + 8.0 * M.nnz + 4.0 * M.nv + (M.blk ? 4.0 * mne : 0.0) +
                          (scheme_ == kSLAK ? 4.0 * M.nnz : 0.0) + 16.0 * L.nv;
-    const double writes = 16.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 16.0 * ne;
+    const double writes = 16.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 16.0 * ne +
+                          (V.out_ptr ? 4.0 * (L.nnz + 1) + 16.0 * L.nnz : 0.0);
     ProfScope ps(st_, "refine_vertex_final_quad", reads + writes);
     launch_final(scheme_, V, st_);
     return;
