@@ -446,6 +446,8 @@ struct RefineView {
   int32_t next_nnz;
 };
 
+constexpr int kMaxVSeg = 8;
+
 // The final level of a single mesh, computed from the structure of the quad
 // mesh M two levels before the output and the positions of its refinement M1
 // (whose structure is never written).
@@ -463,6 +465,11 @@ struct FinalView {
   int32_t* out_ids;     // output quads: 4 per M1 corner, 16 per M corner
   int32_t* out_ptr;     // LAK form of the output (ptr[i] = 4i, vals cyclic 1..4), or null
   int32_t* out_vals;
+  // the vertex items of M as up to kMaxVSeg runs [vseg[s], vseg[s+1]) of
+  // vnb[s] blocks, each run spread in proportion over the launch (nvseg 0: one run)
+  int32_t nvseg;
+  int32_t vseg[kMaxVSeg + 1];
+  int32_t vnb[kMaxVSeg];
 };
 
 // Face, successor and predecessor corner of corner t (quad: bit arithmetic;

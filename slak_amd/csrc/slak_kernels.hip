@@ -1057,16 +1057,35 @@ __device__ __forceinline__ void final_vertex(const FinalView& V, int32_t v) {
   V.out_verts[v] = q;
 }
 
-// corner items and vertex items interleaved in proportion, as in k_level; a
-// corner block stages its 16 KB of output quads in LDS and stores them with
-// consecutive lanes on consecutive 16-byte pieces
+// Corner items and vertex items interleaved in proportion, as in k_level.
+// The vertex blocks are split the same way between the runs of V.vseg (run s
+// against the runs after it): the vertex classes of M (the original vertices,
+// then the face points and edge points of every earlier level), each numbered
+// in the order of the original faces it descends from, so that every run
+// advances over the same part of the surface as the corner blocks and the
+// positions they gather are fetched once for both. A corner block stages its
+// 16 KB of output quads in LDS and stores them with consecutive lanes on
+// consecutive 16-byte pieces.
 template <int SCHEME, bool BND>
 __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int32_t nbv) {
   const int64_t b = block_index();
   const int64_t T = (int64_t)nbc + nbv;
   const int64_t before = b * nbv / T, after = (b + 1) * nbv / T;
   if (after > before) {
-    final_vertex<SCHEME, BND>(V, (int32_t)(before * blockDim.x + threadIdx.x));
+    int64_t u = before, rest = nbv;
+    int s = 0;
+    for (; s + 1 < V.nvseg; ++s) {
+      const int64_t nb = V.vnb[s];
+      const int64_t lo = u * nb / rest, hi = (u + 1) * nb / rest;
+      if (hi > lo) {
+        u = lo;
+        break;
+      }
+      u -= lo;
+      rest -= nb;
+    }
+    const int64_t v = (int64_t)V.vseg[s] + u * blockDim.x + threadIdx.x;
+    final_vertex<SCHEME, BND>(V, v < V.vseg[s + 1] ? (int32_t)v : V.nv);
   } else {
     __shared__ int4 stage[kStageStride * kBlock];
     const int64_t t0 = (b - before) * blockDim.x, t = t0 + threadIdx.x;
@@ -1507,10 +1526,23 @@ void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st) {
   }
 }
 
-void launch_final(int scheme, const FinalView& V, hipStream_t st) {
+void launch_final(int scheme, const FinalView& V0, hipStream_t st) {
+  FinalView V = V0;
   if (V.nnz <= 0 || (V.nnz & 3) || (int64_t)V.nnz != 4 * (int64_t)V.nf)
     throw Error(5, "final level: the mesh two levels back must be quads");  // SLAK_ERR_INTERNAL
-  const int64_t nbc = grid_for(V.nnz), nbv = V.nv > 0 ? grid_for(V.nv) : 0;
+  if (V.nvseg <= 0 || V.nvseg > kMaxVSeg || V.vseg[0] != 0 || V.vseg[V.nvseg] != V.nv) {
+    V.nvseg = 1;
+    V.vseg[0] = 0;
+    V.vseg[1] = V.nv;
+  }
+  int64_t nbv = 0;
+  for (int s = 0; s < V.nvseg; ++s) {
+    const int64_t n = (int64_t)V.vseg[s + 1] - V.vseg[s];
+    if (n < 0) throw Error(5, "final level: vertex runs out of order");  // SLAK_ERR_INTERNAL
+    V.vnb[s] = (int32_t)((n + kBlock - 1) / kBlock);
+    nbv += V.vnb[s];
+  }
+  const int64_t nbc = grid_for(V.nnz);
   if ((nbc + nbv) * kBlock > 0xffffffffll) throw Error(1, "level too large for one launch");
   const unsigned g = (unsigned)(nbc + nbv);
   const bool bnd = V.blk != nullptr;

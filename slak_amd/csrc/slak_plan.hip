@@ -757,6 +757,26 @@ void Plan::record_level(int l) {
     V.out_ids = L.segs[0].dst_ids;
     V.out_ptr = L.segs[0].dst_ptr;
     V.out_vals = L.segs[0].dst_vals;
+    // vertex classes of M: the original vertices, then the face points and
+    // edge points of every level before M (leading ones merged past kMaxVSeg)
+    {
+      std::vector<int64_t> cls{levels_[0].nv};
+      for (int k = 0; k + 2 < l; ++k) {
+        cls.push_back(levels_[k].nf);
+        cls.push_back(levels_[k].ne);
+      }
+      int64_t sum = 0;
+      for (int64_t c : cls) sum += c;
+      if (sum == M.nv) {
+        while (cls.size() > (size_t)kMaxVSeg) {
+          cls[1] += cls[0];
+          cls.erase(cls.begin());
+        }
+        V.nvseg = (int32_t)cls.size();
+        V.vseg[0] = 0;
+        for (size_t i = 0; i < cls.size(); ++i) V.vseg[i + 1] = V.vseg[i] + (int32_t)cls[i];
+      }
+    }
     // algorithmic bytes: reads the ids, (edge, twin) pairs, M^T rows and
     // offsets of M (boundary scan, SLAK positions) and the positions of M1;
     // writes the output quads and every output position
