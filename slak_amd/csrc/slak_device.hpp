@@ -448,25 +448,38 @@ struct RefineView {
 
 constexpr int kMaxVSeg = 8;
 
-// The final level of a single mesh, computed from the structure of the quad
-// mesh M two levels before the output and the positions of its refinement M1
-// (whose structure is never written).
-struct FinalView {
-  const int32_t* ids;   // M corners -> vertices
-  const int2* ci;       // M: (edge id, twin corner or -1) per corner
-  const int32_t* tptr;  // M: M^T row offsets
-  const int2* trow;     // M: M^T rows (corner, in-neighbour), sorted
-  const int32_t* blk;   // M: exclusive scan of its boundary edge flags, or null (closed)
-  const int32_t* mpos;  // M: M^T position per corner (SLAK)
-  int32_t nv, nf, ne;   // M sizes
+// One mesh of a final level computed from the structure of the quad mesh M
+// two levels before the output and the positions of its refinement M1 (whose
+// structure is never written): M's bases in the union arrays of its level,
+// its sizes, its M1 positions and its outputs.
+struct FinalSeg {
+  int64_t cb, vb, eb;    // union bases of M's corners, vertices, edges
+  int32_t nv, nf, ne;    // M sizes
   int32_t nnz;
-  const float4* P1;     // positions of M1 = [M vertices | M face points | M edge points]
-  float4* out_verts;    // output positions: [M1 vertices | M1 face points (already written) | M1 edge points]
-  int32_t* out_ids;     // output quads: 4 per M1 corner, 16 per M corner
-  int32_t* out_ptr;     // LAK form of the output (ptr[i] = 4i, vals cyclic 1..4), or null
+  int32_t id_off;        // added to the output vertex ids (pooled outputs)
+  const float4* P1;      // M1 positions = [M vertices | M face points | M edge points]
+  float4* out_verts;     // output positions: [M1 vertices | M1 face points | M1 edge points]
+  int32_t* out_ids;      // output quads: 4 per M1 corner, 16 per M corner
+  int32_t* out_ptr;      // LAK form of the output (ptr[i] = 4i, vals cyclic 1..4), or null
   int32_t* out_vals;
-  // the vertex items of M as up to kMaxVSeg runs [vseg[s], vseg[s+1]) of
-  // vnb[s] blocks, each run spread in proportion over the launch (nvseg 0: one run)
+};
+
+struct FinalView {
+  // union arrays of M's level
+  const int32_t* ids;    // corners -> vertices
+  const int2* ci;        // (mesh-local edge id, twin corner or -1) per corner
+  const int32_t* tptr;   // M^T row offsets
+  const int2* trow;      // M^T rows (corner, in-neighbour), sorted
+  const int32_t* blk;    // exclusive scan of the boundary edge flags, or null (closed)
+  const int32_t* mpos;   // M^T position per corner (SLAK)
+  FinalSeg one;          // a single mesh (nseg == 1, no table)
+  const FinalSeg* segs;  // several meshes: device table
+  const int64_t* fcb;    // several: first corner item of each mesh (nseg + 1)
+  const int64_t* fvb;    // several: first vertex item of each mesh (nseg + 1)
+  int32_t nseg;
+  int64_t nnz, nv;       // corner and vertex items of the launch
+  // the vertex items as up to kMaxVSeg runs [vseg[s], vseg[s+1]) of vnb[s]
+  // blocks, each run spread in proportion over the launch (nvseg 0: one run)
   int32_t nvseg;
   int32_t vseg[kMaxVSeg + 1];
   int32_t vnb[kMaxVSeg];

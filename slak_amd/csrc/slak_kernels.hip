@@ -42,13 +42,18 @@ constexpr int kRowCap = 8;  // M^T rows up to this length stay in registers
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
 // With the remap, consecutive tiles (which gather from neighbouring parts of
-// the mesh) land on the same XCD. Selected once per process (SLAK_XCD_REMAP=1;
-// off by default: on the 256x128 torus chains it measured 10-25 % slower).
+// the mesh) land on the same XCD: XCD x runs the x-th eighth of the grid in
+// order. Selected once per process by the bit mask SLAK_XCD_REMAP: bit 0 for
+// the level kernels (measured 10-25 % slower on the 256x128 torus chains),
+// bit 1 for k_final (its gathers then stay in one L2: 136 -> 113 us on the
+// 4-level chain of that torus). Default 2.
 __device__ int g_xcd_remap;
+constexpr int kRemapLevel = 1, kRemapFinal = 2;
 
+template <int BIT = kRemapLevel>
 __device__ __forceinline__ int64_t block_index() {
   const int b = blockIdx.x;
-  if (!g_xcd_remap) return b;
+  if (!(g_xcd_remap & BIT)) return b;
   const int nb = gridDim.x;
   const int per = nb >> 3, rem = nb & 7, x = b & 7, i = b >> 3;
   return (int64_t)((x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + i);
@@ -820,41 +825,63 @@ __device__ __forceinline__ float4 face_point(float4 a, float4 b, float4 c, float
 
 constexpr int kStageStride = 5;  // int4 per lane in the staging area (4 used; 5 spreads the banks)
 
-template <int SCHEME, bool BND>
-__device__ __forceinline__ void final_corner(const FinalView& V, int32_t t, int4* __restrict__ stage) {
-  // every lane of a quad runs (nnz = 4 nf and blocks hold whole quads)
+// The mesh of corner item g (one mesh: V.one) and g's mesh-local corner.
+template <bool MULTI>
+__device__ __forceinline__ FinalSeg final_seg(const FinalView& V, int64_t g, const int64_t* base, int32_t& local) {
+  if (!MULTI) {
+    local = (int32_t)g;
+    return V.one;
+  }
+  const int k = seg_of(base, V.nseg, g);
+  local = (int32_t)(g - base[k]);
+  return V.segs[k];
+}
+
+template <int SCHEME, bool BND, bool MULTI>
+__device__ __forceinline__ void final_corner(const FinalView& V, int64_t g, int4* __restrict__ stage,
+                                             int4** __restrict__ sdst, int4** __restrict__ svals) {
+  int32_t t;  // mesh-local corner of M
+  const FinalSeg S = final_seg<MULTI>(V, g, V.fcb, t);
+  // every lane of a quad runs (nnz = 4 nf, meshes and blocks hold whole quads)
+  const int64_t cb = S.cb;
+  const int32_t vb = (int32_t)S.vb;
   const int32_t F = t >> 2, q0 = t & ~3;
   const int32_t tn = q0 | ((t + 1) & 3), tp = q0 | ((t + 3) & 3);
-  const int32_t* __restrict__ ids = V.ids;
-  const int32_t v = ids[t], w = ids[tn], u = ids[tp];
-  const int2 c1 = V.ci[t], c0 = V.ci[tp];
-  const int32_t emax = V.ne > 0 ? V.ne - 1 : 0;
+  const int32_t* __restrict__ ids = V.ids + cb;
+  const int2* __restrict__ ci = V.ci + cb;
+  const int32_t v = ids[t] - vb, w = ids[tn] - vb, u = ids[tp] - vb;
+  const int2 c1 = ci[t], c0 = ci[tp];
+  const int32_t emax = S.ne > 0 ? S.ne - 1 : 0;
   const int32_t e1 = clampi(c1.x, emax), e0 = clampi(c0.x, emax);
-  const int32_t s1 = c1.y, s0 = c0.y;
+  // twins are union corners
+  const int32_t s1 = c1.y >= 0 ? (int32_t)(c1.y - cb) : -1, s0 = c0.y >= 0 ? (int32_t)(c0.y - cb) : -1;
   const int32_t g1 = s1 >= 0 ? (s1 >> 2) : F, g0 = s0 >= 0 ? (s0 >> 2) : F;
   const int32_t n1 = s1 >= 0 ? ((s1 & ~3) | ((s1 + 1) & 3)) : 0;  // next(s1)
   const bool own = s1 < 0 || t < s1;
-  const int32_t nv = V.nv, nnz = V.nnz;
-  const int32_t ep0 = nv + V.nf;     // M1 vertex of the edge point of edge 0 of M
-  const int32_t nv1 = ep0 + V.ne;    // M1 vertices
+  const int32_t nv = S.nv, nnz = S.nnz;
+  const int32_t ep0 = nv + S.nf;     // M1 vertex of the edge point of edge 0 of M
+  const int32_t nv1 = ep0 + S.ne;    // M1 vertices
   const int32_t epb2 = nv1 + nnz;    // output vertex of the edge point of M1 edge 0
-  const float4* __restrict__ P1 = V.P1;
-  float4* __restrict__ out = V.out_verts;
+  const float4* __restrict__ P1 = S.P1;
+  float4* __restrict__ out = S.out_verts;
   int32_t bl1 = 0, bl0 = 0;
   if (BND) {
-    bl1 = V.blk[e1];
-    bl0 = V.blk[e0];
+    const int32_t* __restrict__ blk = V.blk + S.eb;
+    const int32_t bb = blk[0];
+    bl1 = blk[e1] - bb;
+    bl0 = blk[e0] - bb;
   }
-  int32_t mp_t = 0, mp_s0 = 0, mp_s1 = 0;
+  int32_t mp_t = 0, mp_s0 = 0, mp_s1 = 0;  // mesh-local M^T positions
   if (SCHEME == kSLAK) {
-    mp_t = V.mpos[t];
-    if (s0 >= 0) mp_s0 = V.mpos[s0];
-    if (own && s1 >= 0) mp_s1 = V.mpos[s1];
+    const int32_t* __restrict__ mpos = V.mpos + cb;
+    mp_t = mpos[t] - (int32_t)cb;
+    if (s0 >= 0) mp_s0 = mpos[s0] - (int32_t)cb;
+    if (own && s1 >= 0) mp_s1 = mpos[s1] - (int32_t)cb;
   }
   int2 cs0 = make_int2(0, 0), cn1 = make_int2(0, 0);  // (edge, twin) of prev(s1) and next(s1)
   if (own && s1 >= 0) {
-    cs0 = V.ci[(s1 & ~3) | ((s1 + 3) & 3)];
-    cn1 = V.ci[n1];
+    cs0 = ci[(s1 & ~3) | ((s1 + 3) & 3)];
+    cn1 = ci[n1];
   }
   const float4 Pe = P1[ep0 + e1];   // ep(e1)
   const float4 Pv = P1[v];
@@ -895,14 +922,25 @@ __device__ __forceinline__ void final_corner(const FinalView& V, int32_t t, int4
   }
   // the four output quads of M1 face t
   {
-    const int32_t x1 = ep0 + e1, x2 = nv + F, x3 = ep0 + e0, fid = nv1 + t;
-    // into the block's staging area (the block writes them out as 1 KB runs:
-    // 64-byte pieces per lane, stored directly, cost several times more)
-    int4* __restrict__ o = stage + kStageStride * (threadIdx.x & (kBlock - 1));
-    o[0] = make_int4(v, epb2 + ex, fid, epb2 + ew);
-    o[1] = make_int4(x1, epb2 + ey, fid, epb2 + ex);
-    o[2] = make_int4(x2, epb2 + ez, fid, epb2 + ey);
-    o[3] = make_int4(x3, epb2 + ew, fid, epb2 + ez);
+    const int32_t off = S.id_off, eb2 = epb2 + off;
+    const int32_t x0 = v + off, x1 = ep0 + e1 + off, x2 = nv + F + off, x3 = ep0 + e0 + off, fid = nv1 + t + off;
+    // into the block's staging area: the block writes them out as 1 KB runs
+    // (64-byte pieces per lane, stored directly, cost several times more);
+    // several meshes: with each lane's destinations beside them
+    const int lane = threadIdx.x & (kBlock - 1);
+    int4* __restrict__ o = stage + kStageStride * lane;
+    o[0] = make_int4(x0, eb2 + ex, fid, eb2 + ew);
+    o[1] = make_int4(x1, eb2 + ey, fid, eb2 + ex);
+    o[2] = make_int4(x2, eb2 + ez, fid, eb2 + ey);
+    o[3] = make_int4(x3, eb2 + ew, fid, eb2 + ez);
+    if (MULTI) {
+      sdst[lane] = reinterpret_cast<int4*>(S.out_ids) + 4 * (int64_t)t;
+      svals[lane] = S.out_vals ? reinterpret_cast<int4*>(S.out_vals) + 4 * (int64_t)t : nullptr;
+      if (S.out_ptr) {  // LAK form ptr: one int4 per corner of M, 16t + (0, 4, 8, 12)
+        store_nt(S.out_ptr + 4 * (int64_t)t, 16 * t, 16 * t + 4, 16 * t + 8, 16 * t + 12);
+        if (t == nnz - 1) store_nt(S.out_ptr + 4 * (int64_t)nnz, 16 * nnz);
+      }
+    }
   }
   // vertex point of fp(F): row 4t'+2 for the corners t' of F in order,
   // in-neighbours ep(e1(t')); never on the boundary
@@ -983,28 +1021,31 @@ __device__ __forceinline__ void final_corner(const FinalView& V, int32_t t, int4
 // order, faces t, in-neighbours ep(e0(t)); on the boundary 3/4 P1[v] plus 1/8
 // of ep(e1(t)) across every boundary out-edge and of ep(e0(t)) across every
 // boundary in-edge
-template <int SCHEME, bool BND>
-__device__ __forceinline__ void final_vertex(const FinalView& V, int32_t v) {
-  if (v >= V.nv) return;
-  const int32_t ep0 = V.nv + V.nf;
-  const int32_t emax = V.ne > 0 ? V.ne - 1 : 0;
-  const float4* __restrict__ P1 = V.P1;
-  const int2* __restrict__ ci = V.ci;
-  const int32_t p0 = V.tptr[v];
-  const int n = V.tptr[v + 1] - p0;
-  const int2* __restrict__ row = V.trow + p0;
+template <int SCHEME, bool BND, bool MULTI>
+__device__ __forceinline__ void final_vertex(const FinalView& V, int64_t g) {
+  if (g >= V.nv) return;
+  int32_t v;  // mesh-local vertex of M
+  const FinalSeg S = final_seg<MULTI>(V, g, V.fvb, v);
+  const int32_t cb = (int32_t)S.cb;
+  const int32_t ep0 = S.nv + S.nf;
+  const int32_t emax = S.ne > 0 ? S.ne - 1 : 0;
+  const float4* __restrict__ P1 = S.P1;
+  const int2* __restrict__ ci = V.ci + cb;  // mesh-local corners from here on
+  const int32_t p0 = V.tptr[S.vb + v];
+  const int n = V.tptr[S.vb + v + 1] - p0;
+  const int2* __restrict__ row = V.trow + p0;  // union corners
   const float4 Pv = P1[v];
   auto prevc = [](int32_t t) __attribute__((always_inline)) { return (t & ~3) | ((t + 3) & 3); };
   // face point of M1 face t = (v, ep(e1(t)), fp(face of t), ep(e0(t)))
   auto fp_of = [&](int32_t t, int2 c1, float4 pe0) __attribute__((always_inline)) {
-    return face_point(Pv, P1[ep0 + clampi(c1.x, emax)], P1[V.nv + (t >> 2)], pe0);
+    return face_point(Pv, P1[ep0 + clampi(c1.x, emax)], P1[S.nv + (t >> 2)], pe0);
   };
   int32_t tt[kVertexRegs];
   int2 cp[kVertexRegs], co[kVertexRegs];
   float4 a[kVertexRegs], b[kVertexRegs];
 #pragma unroll
   for (int i = 0; i < kVertexRegs; ++i)
-    if (i < n) tt[i] = row[i].x;
+    if (i < n) tt[i] = row[i].x - cb;
 #pragma unroll
   for (int i = 0; i < kVertexRegs; ++i)
     if (i < n) {
@@ -1023,7 +1064,7 @@ __device__ __forceinline__ void final_vertex(const FinalView& V, int32_t v) {
     for (int i = 0; i < kVertexRegs; ++i)
       if (i < n) boundary |= co[i].y < 0 || cp[i].y < 0;
     for (int i = kVertexRegs; i < n; ++i) {
-      const int32_t t = row[i].x;
+      const int32_t t = row[i].x - cb;
       boundary |= ci[t].y < 0 || ci[prevc(t)].y < 0;
     }
   }
@@ -1034,27 +1075,27 @@ __device__ __forceinline__ void final_vertex(const FinalView& V, int32_t v) {
     for (int i = 0; i < kVertexRegs; ++i)
       if (i < n) s = f4add(s, a[i]);
     for (int i = kVertexRegs; i < n; ++i) {
-      const int32_t t = row[i].x;
+      const int32_t t = row[i].x - cb;
       s = f4add(s, fp_of(t, ci[t], P1[ep0 + clampi(ci[prevc(t)].x, emax)]));
     }
 #pragma unroll
     for (int i = 0; i < kVertexRegs; ++i)
       if (i < n) s = f4add(s, b[i]);
-    for (int i = kVertexRegs; i < n; ++i) s = f4add(s, P1[ep0 + clampi(ci[prevc(row[i].x)].x, emax)]);
+    for (int i = kVertexRegs; i < n; ++i) s = f4add(s, P1[ep0 + clampi(ci[prevc(row[i].x - cb)].x, emax)]);
     const float nf = (float)n;
     q = f4div(f4add(f4div(s, nf), f4mul(Pv, nf - 2.0f)), nf);
   } else {
     q = f4mul(Pv, 0.75f);
     for (int i = 0; i < n; ++i) {
-      const int2 c = ci[row[i].x];
+      const int2 c = ci[row[i].x - cb];
       if (c.y < 0) q = f4add(q, f4mul(P1[ep0 + clampi(c.x, emax)], 0.125f));
     }
     for (int i = 0; i < n; ++i) {
-      const int2 c = ci[prevc(row[i].x)];
+      const int2 c = ci[prevc(row[i].x - cb)];
       if (c.y < 0) q = f4add(q, f4mul(P1[ep0 + clampi(c.x, emax)], 0.125f));
     }
   }
-  V.out_verts[v] = q;
+  S.out_verts[v] = q;
 }
 
 // Corner items and vertex items interleaved in proportion, as in k_level.
@@ -1066,9 +1107,9 @@ __device__ __forceinline__ void final_vertex(const FinalView& V, int32_t v) {
 // positions they gather are fetched once for both. A corner block stages its
 // 16 KB of output quads in LDS and stores them with consecutive lanes on
 // consecutive 16-byte pieces.
-template <int SCHEME, bool BND>
+template <int SCHEME, bool BND, bool MULTI>
 __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int32_t nbv) {
-  const int64_t b = block_index();
+  const int64_t b = block_index<kRemapFinal>();
   const int64_t T = (int64_t)nbc + nbv;
   const int64_t before = b * nbv / T, after = (b + 1) * nbv / T;
   if (after > before) {
@@ -1085,14 +1126,35 @@ __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int3
       rest -= nb;
     }
     const int64_t v = (int64_t)V.vseg[s] + u * blockDim.x + threadIdx.x;
-    final_vertex<SCHEME, BND>(V, v < V.vseg[s + 1] ? (int32_t)v : V.nv);
+    final_vertex<SCHEME, BND, MULTI>(V, v < V.vseg[s + 1] ? v : V.nv);
+  } else if (MULTI) {
+    // corners of several meshes: each staged lane carries its own
+    // destinations, consecutive within a mesh
+    __shared__ int4 stage[kStageStride * kBlock];
+    __shared__ int4* sdst[kBlock];
+    __shared__ int4* svals[kBlock];
+    const int64_t t0 = (b - before) * blockDim.x, g = t0 + threadIdx.x;
+    if (g < V.nnz) final_corner<SCHEME, BND, true>(V, g, stage, sdst, svals);
+    __syncthreads();  // the whole block: corner blocks are uniform
+    const int64_t nq = 4 * min((int64_t)kBlock, V.nnz - t0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i = j * kBlock + threadIdx.x;
+      if (i < nq) {
+        const int4 x = stage[kStageStride * (i >> 2) + (i & 3)];
+        int4* const d = sdst[i >> 2] + (i & 3);
+        store_nt(reinterpret_cast<int32_t*>(d), x.x, x.y, x.z, x.w);
+        int4* const a = svals[i >> 2];
+        if (a) store_nt(reinterpret_cast<int32_t*>(a + (i & 3)), 1, 2, 3, 4);
+      }
+    }
   } else {
     __shared__ int4 stage[kStageStride * kBlock];
     const int64_t t0 = (b - before) * blockDim.x, t = t0 + threadIdx.x;
-    if (t < V.nnz) final_corner<SCHEME, BND>(V, (int32_t)t, stage);
+    if (t < V.nnz) final_corner<SCHEME, BND, false>(V, t, stage, nullptr, nullptr);
     __syncthreads();  // the whole block: corner blocks are uniform
     const int64_t nq = 4 * min((int64_t)kBlock, (int64_t)V.nnz - t0);
-    int32_t* __restrict__ dst = V.out_ids + 16 * t0;
+    int32_t* __restrict__ dst = V.one.out_ids + 16 * t0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int i = j * kBlock + threadIdx.x;
@@ -1104,20 +1166,20 @@ __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int3
     // LAK form of the block's 4 x 256 output quads (createQuadColPtr /
     // createQuadVals): every int4 of vals is (1, 2, 3, 4) and int4 q of ptr
     // is 16q + (0, 4, 8, 12), so consecutive lanes take consecutive pieces
-    if (V.out_vals) {
-      int32_t* __restrict__ vals = V.out_vals + 16 * t0;
+    if (V.one.out_vals) {
+      int32_t* __restrict__ vals = V.one.out_vals + 16 * t0;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int i = j * kBlock + threadIdx.x;
         if (i < nq) store_nt(vals + 4 * i, 1, 2, 3, 4);
       }
     }
-    if (V.out_ptr) {
+    if (V.one.out_ptr) {
       if (t < V.nnz) {
         const int32_t q = (int32_t)t;
-        store_nt(V.out_ptr + 4 * (int64_t)q, 16 * q, 16 * q + 4, 16 * q + 8, 16 * q + 12);
+        store_nt(V.one.out_ptr + 4 * (int64_t)q, 16 * q, 16 * q + 4, 16 * q + 8, 16 * q + 12);
       }
-      if (t == (int64_t)V.nnz - 1) store_nt(V.out_ptr + 4 * (int64_t)V.nnz, 16 * V.nnz);
+      if (t == (int64_t)V.nnz - 1) store_nt(V.one.out_ptr + 4 * (int64_t)V.nnz, 16 * (int32_t)V.nnz);
     }
   }
 }
@@ -1375,7 +1437,7 @@ void init_device_options() {
   static std::once_flag once;
   std::call_once(once, [] {
     const char* e = std::getenv("SLAK_XCD_REMAP");
-    int on = e ? std::atoi(e) : 0;
+    int on = e ? std::atoi(e) : kRemapFinal;
     (void)hipMemcpyToSymbol(HIP_SYMBOL(g_xcd_remap), &on, sizeof(int));
   });
 }
@@ -1528,12 +1590,19 @@ void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st) {
 
 void launch_final(int scheme, const FinalView& V0, hipStream_t st) {
   FinalView V = V0;
-  if (V.nnz <= 0 || (V.nnz & 3) || (int64_t)V.nnz != 4 * (int64_t)V.nf)
-    throw Error(5, "final level: the mesh two levels back must be quads");  // SLAK_ERR_INTERNAL
-  if (V.nvseg <= 0 || V.nvseg > kMaxVSeg || V.vseg[0] != 0 || V.vseg[V.nvseg] != V.nv) {
+  const bool multi = V.nseg > 1;
+  if (!multi) {
+    if (V.one.nnz <= 0 || (V.one.nnz & 3) || (int64_t)V.one.nnz != 4 * (int64_t)V.one.nf)
+      throw Error(5, "final level: the mesh two levels back must be quads");  // SLAK_ERR_INTERNAL
+    V.nnz = V.one.nnz;
+    V.nv = V.one.nv;
+  }
+  if (V.nnz <= 0) return;
+  if (V.nv > 0x7fffffffll) throw Error(1, "level too large for one launch");
+  if (multi || V.nvseg <= 0 || V.nvseg > kMaxVSeg || V.vseg[0] != 0 || V.vseg[V.nvseg] != V.nv) {
     V.nvseg = 1;
     V.vseg[0] = 0;
-    V.vseg[1] = V.nv;
+    V.vseg[1] = (int32_t)V.nv;
   }
   int64_t nbv = 0;
   for (int s = 0; s < V.nvseg; ++s) {
@@ -1546,13 +1615,25 @@ void launch_final(int scheme, const FinalView& V0, hipStream_t st) {
   if ((nbc + nbv) * kBlock > 0xffffffffll) throw Error(1, "level too large for one launch");
   const unsigned g = (unsigned)(nbc + nbv);
   const bool bnd = V.blk != nullptr;
+#define SLAK_FINAL(S, B, M) k_final<S, B, M><<<g, kBlock, 0, st>>>(V, (int32_t)nbc, (int32_t)nbv)
   if (scheme == kLAK) {
-    if (bnd) k_final<kLAK, true><<<g, kBlock, 0, st>>>(V, (int32_t)nbc, (int32_t)nbv);
-    else k_final<kLAK, false><<<g, kBlock, 0, st>>>(V, (int32_t)nbc, (int32_t)nbv);
+    if (multi) {
+      if (bnd) SLAK_FINAL(kLAK, true, true);
+      else SLAK_FINAL(kLAK, false, true);
+    } else {
+      if (bnd) SLAK_FINAL(kLAK, true, false);
+      else SLAK_FINAL(kLAK, false, false);
+    }
   } else {
-    if (bnd) k_final<kSLAK, true><<<g, kBlock, 0, st>>>(V, (int32_t)nbc, (int32_t)nbv);
-    else k_final<kSLAK, false><<<g, kBlock, 0, st>>>(V, (int32_t)nbc, (int32_t)nbv);
+    if (multi) {
+      if (bnd) SLAK_FINAL(kSLAK, true, true);
+      else SLAK_FINAL(kSLAK, false, true);
+    } else {
+      if (bnd) SLAK_FINAL(kSLAK, true, false);
+      else SLAK_FINAL(kSLAK, false, false);
+    }
   }
+#undef SLAK_FINAL
 }
 
 void launch_lak_form(const SegTable& seg, int64_t nnz, hipStream_t st) {

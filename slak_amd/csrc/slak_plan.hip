@@ -173,6 +173,9 @@ void Plan::upload_tables(int l0, int l1, Staging& staging, int64_t state_ints) {
   }
   const size_t state_off = total;
   total += align((size_t)state_ints * sizeof(int32_t));
+  const bool fin = l1 == (int)levels_.size() && !final_segs_.empty();
+  const size_t fin_off = total;
+  if (fin) total += align(final_segs_.size() * sizeof(FinalSeg));
   if (total == 0) return;
   if (staging.p) SLAK_HIP_CHECK(hipStreamSynchronize(st_));  // reused: an earlier copy may still read it
   if (staging.cap < total) {
@@ -204,6 +207,10 @@ void Plan::upload_tables(int l0, int l1, Staging& staging, int64_t state_ints) {
       else
         L.d_mflags = arena_.get_n<int>((int64_t)std::max<size_t>(n, 1));
     }
+  }
+  if (fin) {
+    std::memcpy(staging.p + fin_off, final_segs_.data(), final_segs_.size() * sizeof(FinalSeg));
+    d_final_segs_ = reinterpret_cast<FinalSeg*>(d + fin_off);
   }
   SLAK_HIP_CHECK(hipMemcpyAsync(d, staging.p, total, hipMemcpyHostToDevice, st_));
 }
@@ -513,9 +520,11 @@ void Plan::prepare() {
     op += npo;
   }
 
-  // single mesh, two or more levels, quad mesh two levels before the output:
-  // k_final computes the last level (see fused_final_)
-  fused_final_ = jobs_.size() == 1 && Lmax >= 2 && (Lmax >= 3 || levels_[0].quad);
+  // every mesh refined to the same level, two or more, quad meshes two levels
+  // before the output: k_final computes the last level (see fused_final_)
+  bool same_depth = true;
+  for (int j : levels_[0].jobs) same_depth &= jobs_[j].levels == Lmax;
+  fused_final_ = same_depth && Lmax >= 2 && (Lmax >= 3 || levels_[0].quad);
   if (const char* e = std::getenv("SLAK_FUSED_FINAL")) fused_final_ &= std::atoi(e) != 0;
   // level l (1-based) consumes sizes index l-1 and produces index l
   int64_t max_nv = 0, max_nnz = 0;
@@ -685,6 +694,31 @@ void Plan::prepare() {
     L.max_row = l == 1 ? vmax : std::max(std::max(vmax, fmax), 4);
     L.cap = autotune_cap(L.quad, L.nv, L.long_rows, L.max_row);  // used by the first level's row kernels
   }
+  // several meshes: k_final's table of (M, M1, output) per mesh
+  final_segs_.clear();
+  if (fused_final_ && levels_[Lmax - 2].segs.size() > 1) {
+    const LevelPlan& M = levels_[Lmax - 2];
+    const LevelPlan& L = levels_[Lmax - 1];
+    for (size_t k = 0; k < M.segs.size(); ++k) {
+      const SegDesc& m = M.segs[k];
+      const SegDesc& o = L.segs[k];
+      FinalSeg F{};
+      F.cb = m.cb;
+      F.vb = m.vb;
+      F.eb = m.eb;
+      F.nv = m.nv;
+      F.nf = m.nf;
+      F.ne = (int32_t)m.ne;
+      F.nnz = (int32_t)m.nnz;
+      F.id_off = (int32_t)o.id_offset;
+      F.P1 = m.dst_verts;
+      F.out_verts = o.dst_verts;
+      F.out_ids = o.dst_ids;
+      F.out_ptr = o.dst_ptr;
+      F.out_vals = o.dst_vals;
+      final_segs_.push_back(F);
+    }
+  }
   upload_tables(0, (int)levels_.size(), staging_all_);
   prepared_ = true;
 }
@@ -748,15 +782,25 @@ void Plan::record_level(int l) {
     V.trow = M.trow;
     V.blk = M.blk;
     V.mpos = M.mpos;
-    V.nv = (int32_t)M.nv;
-    V.nf = (int32_t)M.nf;
-    V.ne = (int32_t)M.ne;
-    V.nnz = (int32_t)M.nnz;
-    V.P1 = L.P;
-    V.out_verts = L.segs[0].dst_verts;
-    V.out_ids = L.segs[0].dst_ids;
-    V.out_ptr = L.segs[0].dst_ptr;
-    V.out_vals = L.segs[0].dst_vals;
+    V.nnz = M.nnz;
+    V.nv = M.nv;
+    if (M.segs.size() > 1) {
+      V.segs = d_final_segs_;
+      V.fcb = M.d_cbase;
+      V.fvb = M.d_vbase;
+      V.nseg = (int32_t)M.segs.size();
+    } else {
+      V.one.nv = (int32_t)M.nv;
+      V.one.nf = (int32_t)M.nf;
+      V.one.ne = (int32_t)M.ne;
+      V.one.nnz = (int32_t)M.nnz;
+      V.one.P1 = L.P;
+      V.one.out_verts = L.segs[0].dst_verts;
+      V.one.out_ids = L.segs[0].dst_ids;
+      V.one.out_ptr = L.segs[0].dst_ptr;
+      V.one.out_vals = L.segs[0].dst_vals;
+      V.nseg = 1;
+    }
     // vertex classes of M: the original vertices, then the face points and
     // edge points of every level before M (leading ones merged past kMaxVSeg)
     {
@@ -784,7 +828,7 @@ void Plan::record_level(int l) {
     const double reads = 4.0 * M.nnz + 8.0 * M.nnz + 8.0 * M.nnz + 4.0 * M.nv + (M.blk ? 4.0 * mne : 0.0) +
                          (scheme_ == kSLAK ? 4.0 * M.nnz : 0.0) + 16.0 * L.nv;
     const double writes = 16.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 16.0 * ne +
-                          (V.out_ptr ? 4.0 * (L.nnz + 1) + 16.0 * L.nnz : 0.0);
+                          (L.segs[0].dst_ptr ? 4.0 * (L.nnz + 1) + 16.0 * L.nnz : 0.0);
     ProfScope ps(st_, "refine_vertex_final_quad", reads + writes);
     launch_final(scheme_, V, st_);
     return;

@@ -158,10 +158,12 @@ class Plan {
   hipEvent_t order_ev_ = nullptr;
   bool lak_forms_pending_ = false;
   bool lak_forms_in_place_ = false;  // written before a graph capture; replays keep them
-  // single mesh, two or more levels, quad mesh two levels before the output:
-  // the last level is computed from that mesh's structure (k_final) and the
-  // level before it writes no structure
+  // every mesh refined to the same level (two or more), quad meshes two
+  // levels before the output: the last level is computed from their
+  // structure (k_final) and the level before it writes no structure
   bool fused_final_ = false;
+  std::vector<FinalSeg> final_segs_;  // several meshes: k_final's per-mesh table
+  FinalSeg* d_final_segs_ = nullptr;
   // single mesh: the first level's edge numbering ran during the analysis
   // read-back (the level then launches its face points only)
   bool number_early_ = false;

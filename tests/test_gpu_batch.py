@@ -74,6 +74,49 @@ def test_batch_equals_single_mesh_calls(oracle, scheme):
             d.freeAndReset()
 
 
+@pytest.mark.parametrize("scheme", [LAK, SLAK], ids=["LAK", "SLAK"])
+@pytest.mark.parametrize("depth", [2, 3])
+@pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
+def test_same_depth_batch_last_level_in_one_launch(oracle, scheme, depth, use_graph, monkeypatch):
+    """Every mesh refined to the same level: the batch's last level is formed
+    for all meshes at once from the quad meshes two levels back (several
+    meshes per corner block, open and closed, polygonal inputs at depth 3).
+    Each mesh must equal its single-mesh result bit for bit."""
+    hosts, _ = mixed_batch(30, seed=11)
+    if depth == 2:  # two levels: the inputs themselves must be quads
+        hosts = [h for h in hosts if h.type == g.QUAD]
+    hosts = hosts + [g.open_grid(3, 2, sigma=0.01, seed=5, mesh=99), g.torus(5, 4, seed=6, mesh=98)]
+    levels = [depth] * len(hosts)
+    dev = [MeshInfo.from_host(h) for h in hosts]
+    try:
+        with Batch(dev, levels, scheme) as b:
+            for rep in range(2 if use_graph else 1):
+                b.run(use_graph=use_graph)
+                for i, h in enumerate(hosts):
+                    ids, verts, ptr, vals = fetch(b.result(i))
+                    single, _ = subdivide_host(h, depth, scheme)
+                    np.testing.assert_array_equal(ids, single.ids, err_msg=f"mesh {i}")
+                    np.testing.assert_array_equal(verts, single.verts, err_msg=f"mesh {i}")
+                    if scheme == LAK:
+                        np.testing.assert_array_equal(ptr, single.ptr, err_msg=f"mesh {i}")
+                        np.testing.assert_array_equal(vals, single.vals, err_msg=f"mesh {i}")
+                    if rep == 0 and i % 6 == 0:
+                        ref = oracle.subdivide(h, depth, scheme=scheme)
+                        np.testing.assert_array_equal(ids, ref.ids)
+                        assert np.abs(verts - ref.P).max() / h.bbox_diag() <= 1e-5
+            fused = [fetch(b.result(i)) for i in range(len(hosts))]
+        if not use_graph:  # the level-by-level chain gives the same bits
+            monkeypatch.setenv("SLAK_FUSED_FINAL", "0")
+            with Batch(dev, levels, scheme) as b:
+                b.run(use_graph=False)
+                for i in range(len(hosts)):
+                    for got, want in zip(fetch(b.result(i)), fused[i]):
+                        np.testing.assert_array_equal(got, want, err_msg=f"mesh {i}")
+    finally:
+        for d in dev:
+            d.freeAndReset()
+
+
 def test_graph_replays_match_eager_and_track_new_positions():
     hosts, levels = mixed_batch(25, seed=8)
     levels = [max(1, x) for x in levels]
