@@ -166,16 +166,18 @@ void Plan::upload_tables(int l0, int l1, Staging& staging, int64_t state_ints) {
   auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
   size_t total = 0;
   std::vector<size_t> offs;
+  // k_final tables of several meshes: FinalSeg[n], corner and vertex item bases
+  auto fin_bytes = [&](const LevelPlan& L) -> size_t {
+    const size_t n = L.fin.size();
+    return n > 1 ? align(n * sizeof(FinalSeg)) + align(2 * (n + 1) * sizeof(int64_t)) : 0;
+  };
   for (int l = l0; l < l1; ++l) {
     const size_t n = levels_[l].segs.size();
     offs.push_back(total);
-    total = align(total + n * sizeof(SegDesc)) + align(3 * (n + 1) * sizeof(int64_t));
+    total = align(total + n * sizeof(SegDesc)) + align(3 * (n + 1) * sizeof(int64_t)) + fin_bytes(levels_[l]);
   }
   const size_t state_off = total;
   total += align((size_t)state_ints * sizeof(int32_t));
-  const bool fin = l1 == (int)levels_.size() && !final_segs_.empty();
-  const size_t fin_off = total;
-  if (fin) total += align(final_segs_.size() * sizeof(FinalSeg));
   if (total == 0) return;
   if (staging.p) SLAK_HIP_CHECK(hipStreamSynchronize(st_));  // reused: an earlier copy may still read it
   if (staging.cap < total) {
@@ -207,10 +209,17 @@ void Plan::upload_tables(int l0, int l1, Staging& staging, int64_t state_ints) {
       else
         L.d_mflags = arena_.get_n<int>((int64_t)std::max<size_t>(n, 1));
     }
-  }
-  if (fin) {
-    std::memcpy(staging.p + fin_off, final_segs_.data(), final_segs_.size() * sizeof(FinalSeg));
-    d_final_segs_ = reinterpret_cast<FinalSeg*>(d + fin_off);
+    const size_t nf = L.fin.size();
+    if (nf > 1) {
+      o = align(o + 3 * (n + 1) * sizeof(int64_t));
+      std::memcpy(staging.p + o, L.fin.data(), nf * sizeof(FinalSeg));
+      L.d_fin = reinterpret_cast<FinalSeg*>(d + o);
+      o = align(o + nf * sizeof(FinalSeg));
+      std::memcpy(staging.p + o, L.fin_cb.data(), (nf + 1) * sizeof(int64_t));
+      std::memcpy(staging.p + o + (nf + 1) * sizeof(int64_t), L.fin_vb.data(), (nf + 1) * sizeof(int64_t));
+      L.d_fin_cb = reinterpret_cast<int64_t*>(d + o);
+      L.d_fin_vb = L.d_fin_cb + (nf + 1);
+    }
   }
   SLAK_HIP_CHECK(hipMemcpyAsync(d, staging.p, total, hipMemcpyHostToDevice, st_));
 }
@@ -520,12 +529,20 @@ void Plan::prepare() {
     op += npo;
   }
 
-  // every mesh refined to the same level, two or more, quad meshes two levels
-  // before the output: k_final computes the last level (see fused_final_)
-  bool same_depth = true;
-  for (int j : levels_[0].jobs) same_depth &= jobs_[j].levels == Lmax;
-  fused_final_ = same_depth && Lmax >= 2 && (Lmax >= 3 || levels_[0].quad);
-  if (const char* e = std::getenv("SLAK_FUSED_FINAL")) fused_final_ &= std::atoi(e) != 0;
+  // k_final computes the last level of every mesh refined two or more levels
+  // whose mesh two levels before the output is a quad mesh of a quad level
+  // (levels >= 2 are; the first is when every input is a quad mesh)
+  {
+    const char* e = std::getenv("SLAK_FUSED_FINAL");
+    const bool allow = !e || std::atoi(e) != 0;
+    fused_final_ = false;
+    for (int j : levels_[0].jobs) {
+      MeshJob& J = jobs_[j];
+      J.fused = allow && J.levels >= 2 && (J.levels >= 3 || levels_[0].quad);
+      J.p1 = nullptr;
+      fused_final_ |= J.fused;
+    }
+  }
   // level l (1-based) consumes sizes index l-1 and produces index l
   int64_t max_nv = 0, max_nnz = 0;
   levels_.resize(Lmax);
@@ -540,9 +557,14 @@ void Plan::prepare() {
       L.fbase.clear();
       L.cbase.clear();
       int64_t vb = 0, fb = 0, cb = 0, eb = 0;
+      L.fin_jobs.clear();
       for (int j : prev.jobs) {
         MeshJob& J = jobs_[j];
         if (J.levels < l) continue;
+        if (J.fused && J.levels == l) {  // formed by k_final, outside the union
+          L.fin_jobs.push_back(j);
+          continue;
+        }
         SegDesc S{};
         S.vb = vb;
         S.fb = fb;
@@ -573,17 +595,21 @@ void Plan::prepare() {
       max_nv = std::max(max_nv, L.nv);
       max_nnz = std::max(max_nnz, L.nnz);
     }
-    // destinations of this level's segments
+    // destinations of this level's segments; a mesh whose next level is its
+    // fused last one hands on its positions only, outside the next union
+    auto to_final = [l](const MeshJob& J) { return J.fused && J.levels == l + 1; };
     int64_t nv_next = 0, nnz_next = 0;
     for (int j : L.jobs)
-      if (jobs_[j].levels > l) {
+      if (jobs_[j].levels > l && !to_final(jobs_[j])) {
         nv_next += jobs_[j].nv[l];
         nnz_next += jobs_[j].nnz[l];
       }
-    // the level before a fused last level hands on positions only
-    const bool ids_next = !(fused_final_ && l == Lmax - 1);
     float4* Un_P = nv_next ? arena_.get_n<float4>(nv_next) : nullptr;
-    int32_t* Un_ids = nnz_next && ids_next ? arena_.get_n<int32_t>(nnz_next) : nullptr;
+    int32_t* Un_ids = nnz_next ? arena_.get_n<int32_t>(nnz_next) : nullptr;
+    int64_t p1_total = 0, p1_off = 0;  // one block for the positions handed to k_final
+    for (int j : L.jobs)
+      if (to_final(jobs_[j])) p1_total += jobs_[j].nv[l];
+    float4* p1_block = p1_total ? arena_.get_n<float4>(p1_total) : nullptr;
     int64_t vbn = 0, cbn = 0, ebn = 0;
     for (size_t k = 0; k < L.segs.size(); ++k) {
       MeshJob& J = jobs_[L.jobs[k]];
@@ -602,6 +628,17 @@ void Plan::prepare() {
         S.next_cb = -1;
         S.next_eb = -1;
         S.dst_final = 1;
+      } else if (to_final(J)) {
+        J.p1 = p1_block + p1_off;
+        p1_off += J.nv[l];
+        S.dst_verts = J.p1;
+        S.dst_ids = nullptr;
+        S.dst_ptr = nullptr;
+        S.dst_vals = nullptr;
+        S.id_offset = 0;
+        S.next_vb = -1;
+        S.next_cb = -1;
+        S.next_eb = -1;
       } else {
         S.dst_verts = Un_P + vbn;
         S.dst_ids = Un_ids ? Un_ids + cbn : nullptr;
@@ -625,24 +662,13 @@ void Plan::prepare() {
       nx.P = Un_P;
     }
   }
-  if (fused_final_) levels_[Lmax - 1].lak_form = false;  // k_final writes it
-  if (fused_final_)
-    for (SegDesc& S : levels_[Lmax - 2].segs) {  // the level before the last writes no structure
-      S.next_vb = -1;
-      S.next_cb = -1;
-      S.next_eb = -1;
-    }
   // Levels >= 2 receive their structure from the level before (two
   // alternating buffers of each kind, sized for the largest level); the
   // boundary flags and their scan are one buffer pair shared by every level.
-  const int Lstruct = fused_final_ ? Lmax - 1 : Lmax;  // last level whose input structure is written
-  if (fused_final_) {
-    max_nv = max_nnz = 0;
-    for (int l = 2; l <= Lstruct; ++l) {
-      max_nv = std::max(max_nv, levels_[l - 1].nv);
-      max_nnz = std::max(max_nnz, levels_[l - 1].nnz);
-    }
-  }
+  // Only the last level's union can be empty (every mesh in it fused), and
+  // then nothing is written for it.
+  int Lstruct = Lmax;  // last level whose input structure is written
+  while (Lstruct >= 2 && levels_[Lstruct - 1].segs.empty()) --Lstruct;
   if (Lstruct >= 2) {
     int32_t* mt_ptr[2] = {arena_.get_n<int32_t>(max_nv + 1),
                           Lstruct >= 3 ? arena_.get_n<int32_t>(max_nv + 1) : nullptr};
@@ -694,14 +720,22 @@ void Plan::prepare() {
     L.max_row = l == 1 ? vmax : std::max(std::max(vmax, fmax), 4);
     L.cap = autotune_cap(L.quad, L.nv, L.long_rows, L.max_row);  // used by the first level's row kernels
   }
-  // several meshes: k_final's table of (M, M1, output) per mesh
-  final_segs_.clear();
-  if (fused_final_ && levels_[Lmax - 2].segs.size() > 1) {
-    const LevelPlan& M = levels_[Lmax - 2];
-    const LevelPlan& L = levels_[Lmax - 1];
-    for (size_t k = 0; k < M.segs.size(); ++k) {
+  // k_final tables: per fused mesh, its segment of M (the level before's
+  // union), the positions of M1 and its outputs
+  std::vector<int> seg_in_m(jobs_.size(), -1);  // job -> its segment of M
+  for (int l = 2; l <= Lmax; ++l) {
+    LevelPlan& L = levels_[l - 1];
+    const LevelPlan& M = levels_[l - 2];
+    L.fin.clear();
+    L.fin_cb.assign(1, 0);
+    L.fin_vb.assign(1, 0);
+    if (L.fin_jobs.empty()) continue;
+    for (size_t k = 0; k < M.jobs.size(); ++k) seg_in_m[M.jobs[k]] = (int)k;
+    for (int j : L.fin_jobs) {
+      const MeshJob& J = jobs_[j];
+      const int k = seg_in_m[j];
+      if (k < 0 || !J.p1) throw Error(SLAK_ERR_INTERNAL, "fused level without its input");
       const SegDesc& m = M.segs[k];
-      const SegDesc& o = L.segs[k];
       FinalSeg F{};
       F.cb = m.cb;
       F.vb = m.vb;
@@ -710,13 +744,15 @@ void Plan::prepare() {
       F.nf = m.nf;
       F.ne = (int32_t)m.ne;
       F.nnz = (int32_t)m.nnz;
-      F.id_off = (int32_t)o.id_offset;
-      F.P1 = m.dst_verts;
-      F.out_verts = o.dst_verts;
-      F.out_ids = o.dst_ids;
-      F.out_ptr = o.dst_ptr;
-      F.out_vals = o.dst_vals;
-      final_segs_.push_back(F);
+      F.id_off = 0;
+      F.P1 = J.p1;
+      F.out_verts = J.out_verts;
+      F.out_ids = J.out_ids;
+      F.out_ptr = J.out_ptr;
+      F.out_vals = J.out_vals;
+      L.fin.push_back(F);
+      L.fin_cb.push_back(L.fin_cb.back() + m.nnz);
+      L.fin_vb.push_back(L.fin_vb.back() + m.nv);
     }
   }
   upload_tables(0, (int)levels_.size(), staging_all_);
@@ -750,8 +786,70 @@ void Plan::launch_number(LevelPlan& L, bool faces) {
   launch_number_faces(L.quad, scheme_, L.cap, V, faces, st_);
 }
 
+// The last level of the meshes fused at level l (k_final), from the previous
+// level's structure: launched before anything of level l overwrites it (the
+// boundary scan buffer, the alternating M^T buffers).
+void Plan::record_final(int l) {
+  LevelPlan& L = levels_[l - 1];
+  if (L.fin.empty()) return;
+  const LevelPlan& M = levels_[l - 2];
+  FinalView V{};
+  V.ids = M.ids;
+  V.ci = M.ci;
+  V.tptr = M.tptr;
+  V.trow = M.trow;
+  V.blk = M.blk;
+  V.mpos = M.mpos;
+  V.nnz = L.fin_cb.back();
+  V.nv = L.fin_vb.back();
+  if (L.fin.size() > 1) {
+    V.segs = L.d_fin;
+    V.fcb = L.d_fin_cb;
+    V.fvb = L.d_fin_vb;
+    V.nseg = (int32_t)L.fin.size();
+  } else {
+    V.one = L.fin[0];
+    V.nseg = 1;
+    // vertex classes of M: the original vertices, then the face points and
+    // edge points of every level before M (leading ones merged past kMaxVSeg)
+    const MeshJob& J = jobs_[L.fin_jobs[0]];
+    std::vector<int64_t> cls{J.nv[0]};
+    for (int k = 0; k + 2 < l; ++k) {
+      cls.push_back(J.nf[k]);
+      cls.push_back(J.ne[k]);
+    }
+    int64_t sum = 0;
+    for (int64_t c : cls) sum += c;
+    if (sum == J.nv[l - 2]) {
+      while (cls.size() > (size_t)kMaxVSeg) {
+        cls[1] += cls[0];
+        cls.erase(cls.begin());
+      }
+      V.nvseg = (int32_t)cls.size();
+      V.vseg[0] = 0;
+      for (size_t i = 0; i < cls.size(); ++i) V.vseg[i + 1] = V.vseg[i] + (int32_t)cls[i];
+    }
+  }
+  // algorithmic bytes: reads the ids, (edge, twin) pairs, M^T rows and
+  // offsets of M (boundary scan, SLAK positions) and the positions of M1;
+  // writes the output quads and every output position (and the LAK form)
+  double reads = 0, writes = 0;
+  for (int j : L.fin_jobs) {
+    const MeshJob& J = jobs_[j];
+    const double mnnz = (double)J.nnz[l - 2], mnv = (double)J.nv[l - 2], mne = (double)J.ne[l - 2];
+    const double nnz1 = (double)J.nnz[l - 1], nv1 = (double)J.nv[l - 1], nf1 = (double)J.nf[l - 1];
+    const double ne1 = (double)J.ne[l - 1];
+    reads += 20.0 * mnnz + 4.0 * mnv + (M.blk ? 4.0 * mne : 0.0) + (scheme_ == kSLAK ? 4.0 * mnnz : 0.0) + 16.0 * nv1;
+    writes += 16.0 * nnz1 + 16.0 * nv1 + 16.0 * nf1 + 16.0 * ne1 + (J.out_ptr ? 4.0 * (nnz1 + 1) + 16.0 * nnz1 : 0.0);
+  }
+  ProfScope ps(st_, "refine_vertex_final_quad", reads + writes);
+  launch_final(scheme_, V, st_);
+}
+
 void Plan::record_level(int l) {
   LevelPlan& L = levels_[l - 1];
+  if (l > 1) record_final(l);
+  if (L.segs.empty()) return;  // every mesh of the level was fused
   const double poly_b = L.quad ? 0.0 : 4.0 * (L.nf + 1) + 4.0 * L.nnz;
   const double ne = (double)L.ne;
   if (l == 1 && !number_early_) {
@@ -767,71 +865,10 @@ void Plan::record_level(int l) {
     ProfScope ps(st_, "scan", 8.0 * ne + 4);
     launch_scan(L.bnd, L.blk, L.ne, d_partial_, d_flags_, st_);
   }
-  const bool fused_last = fused_final_ && l == (int)levels_.size();  // k_final forms the face points
-  if (l > 1 && !fused_last) {
+  if (l > 1) {
     ProfScope ps(st_, L.quad ? "facepoints_quad" : "facepoints_poly",
                  4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + (L.quad ? 0.0 : 4.0 * (L.nf + 1)));
     launch_faces(L.quad, true, false, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), nullptr, nullptr, st_);
-  }
-  if (fused_last) {
-    const LevelPlan& M = levels_[l - 2];
-    FinalView V{};
-    V.ids = M.ids;
-    V.ci = M.ci;
-    V.tptr = M.tptr;
-    V.trow = M.trow;
-    V.blk = M.blk;
-    V.mpos = M.mpos;
-    V.nnz = M.nnz;
-    V.nv = M.nv;
-    if (M.segs.size() > 1) {
-      V.segs = d_final_segs_;
-      V.fcb = M.d_cbase;
-      V.fvb = M.d_vbase;
-      V.nseg = (int32_t)M.segs.size();
-    } else {
-      V.one.nv = (int32_t)M.nv;
-      V.one.nf = (int32_t)M.nf;
-      V.one.ne = (int32_t)M.ne;
-      V.one.nnz = (int32_t)M.nnz;
-      V.one.P1 = L.P;
-      V.one.out_verts = L.segs[0].dst_verts;
-      V.one.out_ids = L.segs[0].dst_ids;
-      V.one.out_ptr = L.segs[0].dst_ptr;
-      V.one.out_vals = L.segs[0].dst_vals;
-      V.nseg = 1;
-    }
-    // vertex classes of M: the original vertices, then the face points and
-    // edge points of every level before M (leading ones merged past kMaxVSeg)
-    {
-      std::vector<int64_t> cls{levels_[0].nv};
-      for (int k = 0; k + 2 < l; ++k) {
-        cls.push_back(levels_[k].nf);
-        cls.push_back(levels_[k].ne);
-      }
-      int64_t sum = 0;
-      for (int64_t c : cls) sum += c;
-      if (sum == M.nv) {
-        while (cls.size() > (size_t)kMaxVSeg) {
-          cls[1] += cls[0];
-          cls.erase(cls.begin());
-        }
-        V.nvseg = (int32_t)cls.size();
-        V.vseg[0] = 0;
-        for (size_t i = 0; i < cls.size(); ++i) V.vseg[i + 1] = V.vseg[i] + (int32_t)cls[i];
-      }
-    }
-    // algorithmic bytes: reads the ids, (edge, twin) pairs, M^T rows and
-    // offsets of M (boundary scan, SLAK positions) and the positions of M1;
-    // writes the output quads and every output position
-    const double mne = (double)M.ne;
-    const double reads = 4.0 * M.nnz + 8.0 * M.nnz + 8.0 * M.nnz + 4.0 * M.nv + (M.blk ? 4.0 * mne : 0.0) +
-                         (scheme_ == kSLAK ? 4.0 * M.nnz : 0.0) + 16.0 * L.nv;
-    const double writes = 16.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 16.0 * ne +
-                          (L.segs[0].dst_ptr ? 4.0 * (L.nnz + 1) + 16.0 * L.nnz : 0.0);
-    ProfScope ps(st_, "refine_vertex_final_quad", reads + writes);
-    launch_final(scheme_, V, st_);
-    return;
   }
   RefineView R{};
   R.ids = L.ids;

@@ -36,6 +36,10 @@ struct MeshJob {
   int32_t* out_ptr = nullptr;
   int32_t* out_vals = nullptr;
   int bad_flags = 0;
+  // the last level is formed by k_final from the quad mesh two levels back;
+  // p1: the positions of the level before the output (its only output)
+  bool fused = false;
+  float4* p1 = nullptr;
 };
 
 struct LevelPlan {
@@ -70,6 +74,15 @@ struct LevelPlan {
   std::vector<SegDesc> segs;
   std::vector<int64_t> vbase, fbase, cbase;
   bool lak_form = false;  // some mesh finishes here with the LAK form (ptr / vals)
+  // meshes whose last level is this one, formed by k_final from the previous
+  // level's structure (they are not segments of this level's union):
+  // per-mesh table and item bases (corners and vertices of M, nfin + 1)
+  std::vector<int> fin_jobs;
+  std::vector<FinalSeg> fin;
+  std::vector<int64_t> fin_cb, fin_vb;
+  FinalSeg* d_fin = nullptr;
+  int64_t* d_fin_cb = nullptr;
+  int64_t* d_fin_vb = nullptr;
   SegDesc* d_segs = nullptr;
   int64_t* d_vbase = nullptr;
   int64_t* d_fbase = nullptr;
@@ -158,12 +171,11 @@ class Plan {
   hipEvent_t order_ev_ = nullptr;
   bool lak_forms_pending_ = false;
   bool lak_forms_in_place_ = false;  // written before a graph capture; replays keep them
-  // every mesh refined to the same level (two or more), quad meshes two
-  // levels before the output: the last level is computed from their
-  // structure (k_final) and the level before it writes no structure
+  // some mesh refined two or more levels with a quad mesh two levels before
+  // its output: its last level is computed from that mesh's structure
+  // (k_final) and the level before it writes positions only
   bool fused_final_ = false;
-  std::vector<FinalSeg> final_segs_;  // several meshes: k_final's per-mesh table
-  FinalSeg* d_final_segs_ = nullptr;
+  void record_final(int l);
   // single mesh: the first level's edge numbering ran during the analysis
   // read-back (the level then launches its face points only)
   bool number_early_ = false;

@@ -45,10 +45,18 @@ def mixed_batch(n, seed):
 
 
 @pytest.mark.parametrize("scheme", [LAK, SLAK], ids=["LAK", "SLAK"])
-def test_batch_equals_single_mesh_calls(oracle, scheme):
+def test_batch_equals_single_mesh_calls(oracle, scheme, monkeypatch):
+    """Mixed depths 0-3: the meshes whose last level is formed from the quad
+    mesh two levels back are taken out of that level's union and formed
+    together in one launch; the results must not depend on it."""
     hosts, levels = mixed_batch(40, seed=3)
     dev = [MeshInfo.from_host(h) for h in hosts]
     try:
+        monkeypatch.setenv("SLAK_FUSED_FINAL", "0")
+        with Batch(dev, levels, scheme) as b:
+            b.run(use_graph=False)
+            chain = [None if r.ids == 0 else fetch(r) for r in (b.result(i) for i in range(len(hosts)))]
+        monkeypatch.delenv("SLAK_FUSED_FINAL")
         with Batch(dev, levels, scheme) as b:
             b.run(use_graph=False)
             for i, (h, lv) in enumerate(zip(hosts, levels)):
@@ -57,6 +65,8 @@ def test_batch_equals_single_mesh_calls(oracle, scheme):
                     assert r.ids == 0  # untouched, as subdivideIteratively leaves it
                     continue
                 ids, verts, ptr, vals = fetch(r)
+                for got, want in zip((ids, verts, ptr, vals), chain[i]):
+                    np.testing.assert_array_equal(got, want, err_msg=f"mesh {i}")
                 single, _ = subdivide_host(h, lv, scheme)
                 np.testing.assert_array_equal(ids, single.ids, err_msg=f"mesh {i}")
                 np.testing.assert_array_equal(verts, single.verts, err_msg=f"mesh {i}")
