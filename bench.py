@@ -209,6 +209,7 @@ def batched_vs_plain(n_meshes, steps, scheme, seed=4):
         "faces_per_s_batched": round(faces / batched_s, 1),
         "faces_per_s_plain": round(faces / plain_s, 1),
         "roofline": rl,
+        "stages": prof["stages"],
         "generation_s": round(gen_s, 2),
     }
 

@@ -452,6 +452,8 @@ constexpr int kMaxVSeg = 8;
 // two levels before the output and the positions of its refinement M1 (whose
 // structure is never written): M's bases in the union arrays of its level,
 // its sizes, its M1 positions and its outputs.
+constexpr int kBlockItems = 256;  // threads (items) per block of the level kernels
+
 struct FinalSeg {
   int64_t cb, vb, eb;    // union bases of M's corners, vertices, edges
   int32_t nv, nf, ne;    // M sizes
@@ -474,8 +476,13 @@ struct FinalView {
   const int32_t* mpos;   // M^T position per corner (SLAK)
   FinalSeg one;          // a single mesh (nseg == 1, no table)
   const FinalSeg* segs;  // several meshes: device table
-  const int64_t* fcb;    // several: first corner item of each mesh (nseg + 1)
-  const int64_t* fvb;    // several: first vertex item of each mesh (nseg + 1)
+  // several meshes: work units of one block each, (mesh, first mesh-local
+  // item, count, 0): corner units of consecutive corners of one mesh in
+  // mesh order; vertex units of consecutive vertices of one class of one
+  // mesh, each mesh's vertex classes interleaved in proportion
+  const int4* cunits;
+  const int4* vunits;
+  int32_t ncunits, nvunits;
   int32_t nseg;
   int64_t nnz, nv;       // corner and vertex items of the launch
   // the vertex items as up to kMaxVSeg runs [vseg[s], vseg[s+1]) of vnb[s]

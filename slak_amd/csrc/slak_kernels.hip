@@ -37,7 +37,7 @@ namespace slak {
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = kBlockItems;
 constexpr int kRowCap = 8;  // M^T rows up to this length stay in registers
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
@@ -48,7 +48,7 @@ constexpr int kRowCap = 8;  // M^T rows up to this length stay in registers
 // bit 1 for k_final (its gathers then stay in one L2: 136 -> 113 us on the
 // 4-level chain of that torus). Default 2.
 __device__ int g_xcd_remap;
-constexpr int kRemapLevel = 1, kRemapFinal = 2;
+constexpr int kRemapLevel = 1, kRemapFinal = 2, kRemapPositions = 4;
 
 template <int BIT = kRemapLevel>
 __device__ __forceinline__ int64_t block_index() {
@@ -764,7 +764,7 @@ __device__ __forceinline__ void vertex_item(const RefineView& R, int32_t v) {
 // b - floor(b nbv / T); a bijection onto both block ranges.
 template <bool QUAD, int SCHEME, bool NEXT, bool BND>
 __global__ void __launch_bounds__(kBlock) k_level(RefineView R, int32_t nbr, int32_t nbv) {
-  const int64_t b = block_index();
+  const int64_t b = NEXT ? block_index<kRemapLevel>() : block_index<kRemapPositions>();
   const int64_t T = (int64_t)nbr + nbv;
   const int64_t before = b * nbv / T, after = (b + 1) * nbv / T;
   if (after > before)
@@ -825,23 +825,11 @@ __device__ __forceinline__ float4 face_point(float4 a, float4 b, float4 c, float
 
 constexpr int kStageStride = 5;  // int4 per lane in the staging area (4 used; 5 spreads the banks)
 
-// The mesh of corner item g (one mesh: V.one) and g's mesh-local corner.
-template <bool MULTI>
-__device__ __forceinline__ FinalSeg final_seg(const FinalView& V, int64_t g, const int64_t* base, int32_t& local) {
-  if (!MULTI) {
-    local = (int32_t)g;
-    return V.one;
-  }
-  const int k = seg_of(base, V.nseg, g);
-  local = (int32_t)(g - base[k]);
-  return V.segs[k];
-}
-
-template <int SCHEME, bool BND, bool MULTI>
-__device__ __forceinline__ void final_corner(const FinalView& V, int64_t g, int4* __restrict__ stage,
-                                             int4** __restrict__ sdst, int4** __restrict__ svals) {
-  int32_t t;  // mesh-local corner of M
-  const FinalSeg S = final_seg<MULTI>(V, g, V.fcb, t);
+// Corner t (mesh-local) of M in mesh S; its 16 output ids go to the block's
+// staging area.
+template <int SCHEME, bool BND>
+__device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg& S, int32_t t,
+                                             int4* __restrict__ stage) {
   // every lane of a quad runs (nnz = 4 nf, meshes and blocks hold whole quads)
   const int64_t cb = S.cb;
   const int32_t vb = (int32_t)S.vb;
@@ -925,22 +913,12 @@ __device__ __forceinline__ void final_corner(const FinalView& V, int64_t g, int4
     const int32_t off = S.id_off, eb2 = epb2 + off;
     const int32_t x0 = v + off, x1 = ep0 + e1 + off, x2 = nv + F + off, x3 = ep0 + e0 + off, fid = nv1 + t + off;
     // into the block's staging area: the block writes them out as 1 KB runs
-    // (64-byte pieces per lane, stored directly, cost several times more);
-    // several meshes: with each lane's destinations beside them
-    const int lane = threadIdx.x & (kBlock - 1);
-    int4* __restrict__ o = stage + kStageStride * lane;
+    // (64-byte pieces per lane, stored directly, cost several times more)
+    int4* __restrict__ o = stage + kStageStride * (threadIdx.x & (kBlock - 1));
     o[0] = make_int4(x0, eb2 + ex, fid, eb2 + ew);
     o[1] = make_int4(x1, eb2 + ey, fid, eb2 + ex);
     o[2] = make_int4(x2, eb2 + ez, fid, eb2 + ey);
     o[3] = make_int4(x3, eb2 + ew, fid, eb2 + ez);
-    if (MULTI) {
-      sdst[lane] = reinterpret_cast<int4*>(S.out_ids) + 4 * (int64_t)t;
-      svals[lane] = S.out_vals ? reinterpret_cast<int4*>(S.out_vals) + 4 * (int64_t)t : nullptr;
-      if (S.out_ptr) {  // LAK form ptr: one int4 per corner of M, 16t + (0, 4, 8, 12)
-        store_nt(S.out_ptr + 4 * (int64_t)t, 16 * t, 16 * t + 4, 16 * t + 8, 16 * t + 12);
-        if (t == nnz - 1) store_nt(S.out_ptr + 4 * (int64_t)nnz, 16 * nnz);
-      }
-    }
   }
   // vertex point of fp(F): row 4t'+2 for the corners t' of F in order,
   // in-neighbours ep(e1(t')); never on the boundary
@@ -1021,11 +999,8 @@ __device__ __forceinline__ void final_corner(const FinalView& V, int64_t g, int4
 // order, faces t, in-neighbours ep(e0(t)); on the boundary 3/4 P1[v] plus 1/8
 // of ep(e1(t)) across every boundary out-edge and of ep(e0(t)) across every
 // boundary in-edge
-template <int SCHEME, bool BND, bool MULTI>
-__device__ __forceinline__ void final_vertex(const FinalView& V, int64_t g) {
-  if (g >= V.nv) return;
-  int32_t v;  // mesh-local vertex of M
-  const FinalSeg S = final_seg<MULTI>(V, g, V.fvb, v);
+template <int SCHEME, bool BND>
+__device__ __forceinline__ void final_vertex(const FinalView& V, const FinalSeg& S, int32_t v) {
   const int32_t cb = (int32_t)S.cb;
   const int32_t ep0 = S.nv + S.nf;
   const int32_t emax = S.ne > 0 ? S.ne - 1 : 0;
@@ -1113,6 +1088,11 @@ __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int3
   const int64_t T = (int64_t)nbc + nbv;
   const int64_t before = b * nbv / T, after = (b + 1) * nbv / T;
   if (after > before) {
+    if (MULTI) {  // work unit `before`: consecutive vertices of one class of one mesh
+      const int4 U = V.vunits[before];
+      if ((int)threadIdx.x < U.z) final_vertex<SCHEME, BND>(V, V.segs[U.x], U.y + (int32_t)threadIdx.x);
+      return;
+    }
     int64_t u = before, rest = nbv;
     int s = 0;
     for (; s + 1 < V.nvseg; ++s) {
@@ -1126,35 +1106,29 @@ __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int3
       rest -= nb;
     }
     const int64_t v = (int64_t)V.vseg[s] + u * blockDim.x + threadIdx.x;
-    final_vertex<SCHEME, BND, MULTI>(V, v < V.vseg[s + 1] ? v : V.nv);
-  } else if (MULTI) {
-    // corners of several meshes: each staged lane carries its own
-    // destinations, consecutive within a mesh
-    __shared__ int4 stage[kStageStride * kBlock];
-    __shared__ int4* sdst[kBlock];
-    __shared__ int4* svals[kBlock];
-    const int64_t t0 = (b - before) * blockDim.x, g = t0 + threadIdx.x;
-    if (g < V.nnz) final_corner<SCHEME, BND, true>(V, g, stage, sdst, svals);
-    __syncthreads();  // the whole block: corner blocks are uniform
-    const int64_t nq = 4 * min((int64_t)kBlock, V.nnz - t0);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int i = j * kBlock + threadIdx.x;
-      if (i < nq) {
-        const int4 x = stage[kStageStride * (i >> 2) + (i & 3)];
-        int4* const d = sdst[i >> 2] + (i & 3);
-        store_nt(reinterpret_cast<int32_t*>(d), x.x, x.y, x.z, x.w);
-        int4* const a = svals[i >> 2];
-        if (a) store_nt(reinterpret_cast<int32_t*>(a + (i & 3)), 1, 2, 3, 4);
-      }
-    }
+    if (v < V.vseg[s + 1]) final_vertex<SCHEME, BND>(V, V.one, (int32_t)v);
   } else {
+    // one mesh per corner block: several meshes take it from the block's
+    // corner unit (block-uniform, like the single mesh's descriptor)
     __shared__ int4 stage[kStageStride * kBlock];
-    const int64_t t0 = (b - before) * blockDim.x, t = t0 + threadIdx.x;
-    if (t < V.nnz) final_corner<SCHEME, BND, false>(V, t, stage, nullptr, nullptr);
+    const int64_t cblk = b - before;
+    FinalSeg S;
+    int32_t t0, cnt;
+    if (MULTI) {
+      const int4 U = V.cunits[cblk];
+      S = V.segs[U.x];
+      t0 = U.y;
+      cnt = U.z;
+    } else {
+      S = V.one;
+      t0 = (int32_t)(cblk * kBlock);
+      cnt = (int32_t)min((int64_t)kBlock, V.nnz - t0);
+    }
+    const int32_t t = t0 + (int32_t)threadIdx.x;
+    if ((int)threadIdx.x < cnt) final_corner<SCHEME, BND>(V, S, t, stage);
     __syncthreads();  // the whole block: corner blocks are uniform
-    const int64_t nq = 4 * min((int64_t)kBlock, (int64_t)V.nnz - t0);
-    int32_t* __restrict__ dst = V.one.out_ids + 16 * t0;
+    const int nq = 4 * cnt;
+    int32_t* __restrict__ dst = S.out_ids + 16 * (int64_t)t0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int i = j * kBlock + threadIdx.x;
@@ -1166,20 +1140,17 @@ __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int3
     // LAK form of the block's 4 x 256 output quads (createQuadColPtr /
     // createQuadVals): every int4 of vals is (1, 2, 3, 4) and int4 q of ptr
     // is 16q + (0, 4, 8, 12), so consecutive lanes take consecutive pieces
-    if (V.one.out_vals) {
-      int32_t* __restrict__ vals = V.one.out_vals + 16 * t0;
+    if (S.out_vals) {
+      int32_t* __restrict__ vals = S.out_vals + 16 * (int64_t)t0;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int i = j * kBlock + threadIdx.x;
         if (i < nq) store_nt(vals + 4 * i, 1, 2, 3, 4);
       }
     }
-    if (V.one.out_ptr) {
-      if (t < V.nnz) {
-        const int32_t q = (int32_t)t;
-        store_nt(V.one.out_ptr + 4 * (int64_t)q, 16 * q, 16 * q + 4, 16 * q + 8, 16 * q + 12);
-      }
-      if (t == (int64_t)V.nnz - 1) store_nt(V.one.out_ptr + 4 * (int64_t)V.nnz, 16 * (int32_t)V.nnz);
+    if (S.out_ptr) {
+      if ((int)threadIdx.x < cnt) store_nt(S.out_ptr + 4 * (int64_t)t, 16 * t, 16 * t + 4, 16 * t + 8, 16 * t + 12);
+      if (t == S.nnz - 1) store_nt(S.out_ptr + 4 * (int64_t)S.nnz, 16 * S.nnz);
     }
   }
 }
@@ -1599,7 +1570,11 @@ void launch_final(int scheme, const FinalView& V0, hipStream_t st) {
   }
   if (V.nnz <= 0) return;
   if (V.nv > 0x7fffffffll) throw Error(1, "level too large for one launch");
-  if (multi || V.nvseg <= 0 || V.nvseg > kMaxVSeg || V.vseg[0] != 0 || V.vseg[V.nvseg] != V.nv) {
+  static const bool runs = [] {  // SLAK_FINAL_RUNS=0: vertex items in plain order
+    const char* e = std::getenv("SLAK_FINAL_RUNS");
+    return !e || std::atoi(e) != 0;
+  }();
+  if (multi || !runs || V.nvseg <= 0 || V.nvseg > kMaxVSeg || V.vseg[0] != 0 || V.vseg[V.nvseg] != V.nv) {
     V.nvseg = 1;
     V.vseg[0] = 0;
     V.vseg[1] = (int32_t)V.nv;
@@ -1611,7 +1586,13 @@ void launch_final(int scheme, const FinalView& V0, hipStream_t st) {
     V.vnb[s] = (int32_t)((n + kBlock - 1) / kBlock);
     nbv += V.vnb[s];
   }
-  const int64_t nbc = grid_for(V.nnz);
+  int64_t nbc = grid_for(V.nnz);
+  if (multi) {
+    if (!V.cunits || !V.vunits || V.ncunits <= 0 || V.nvunits <= 0)
+      throw Error(5, "final level: no work units");  // SLAK_ERR_INTERNAL
+    nbc = V.ncunits;
+    nbv = V.nvunits;
+  }
   if ((nbc + nbv) * kBlock > 0xffffffffll) throw Error(1, "level too large for one launch");
   const unsigned g = (unsigned)(nbc + nbv);
   const bool bnd = V.blk != nullptr;

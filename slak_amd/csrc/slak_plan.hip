@@ -169,7 +169,9 @@ void Plan::upload_tables(int l0, int l1, Staging& staging, int64_t state_ints) {
   // k_final tables of several meshes: FinalSeg[n], corner and vertex item bases
   auto fin_bytes = [&](const LevelPlan& L) -> size_t {
     const size_t n = L.fin.size();
-    return n > 1 ? align(n * sizeof(FinalSeg)) + align(2 * (n + 1) * sizeof(int64_t)) : 0;
+    return n > 1 ? align(n * sizeof(FinalSeg)) + align(L.fin_cunits.size() * sizeof(int4)) +
+                       align(L.fin_vunits.size() * sizeof(int4))
+                 : 0;
   };
   for (int l = l0; l < l1; ++l) {
     const size_t n = levels_[l].segs.size();
@@ -215,10 +217,11 @@ void Plan::upload_tables(int l0, int l1, Staging& staging, int64_t state_ints) {
       std::memcpy(staging.p + o, L.fin.data(), nf * sizeof(FinalSeg));
       L.d_fin = reinterpret_cast<FinalSeg*>(d + o);
       o = align(o + nf * sizeof(FinalSeg));
-      std::memcpy(staging.p + o, L.fin_cb.data(), (nf + 1) * sizeof(int64_t));
-      std::memcpy(staging.p + o + (nf + 1) * sizeof(int64_t), L.fin_vb.data(), (nf + 1) * sizeof(int64_t));
-      L.d_fin_cb = reinterpret_cast<int64_t*>(d + o);
-      L.d_fin_vb = L.d_fin_cb + (nf + 1);
+      std::memcpy(staging.p + o, L.fin_cunits.data(), L.fin_cunits.size() * sizeof(int4));
+      L.d_fin_cunits = reinterpret_cast<int4*>(d + o);
+      o = align(o + L.fin_cunits.size() * sizeof(int4));
+      std::memcpy(staging.p + o, L.fin_vunits.data(), L.fin_vunits.size() * sizeof(int4));
+      L.d_fin_vunits = reinterpret_cast<int4*>(d + o);
     }
   }
   SLAK_HIP_CHECK(hipMemcpyAsync(d, staging.p, total, hipMemcpyHostToDevice, st_));
@@ -754,6 +757,7 @@ void Plan::prepare() {
       L.fin_cb.push_back(L.fin_cb.back() + m.nnz);
       L.fin_vb.push_back(L.fin_vb.back() + m.nv);
     }
+    if (L.fin.size() > 1) build_final_units(l);
   }
   upload_tables(0, (int)levels_.size(), staging_all_);
   prepared_ = true;
@@ -786,6 +790,59 @@ void Plan::launch_number(LevelPlan& L, bool faces) {
   launch_number_faces(L.quad, scheme_, L.cap, V, faces, st_);
 }
 
+// Vertex classes of the mesh two levels before job J's level l (the mesh M
+// of its k_final): the original vertices, then the face points and edge
+// points of every level before M; one class if the sizes do not add up.
+static std::vector<int64_t> vertex_classes(const MeshJob& J, int l) {
+  std::vector<int64_t> cls{J.nv[0]};
+  for (int k = 0; k + 2 < l; ++k) {
+    cls.push_back(J.nf[k]);
+    cls.push_back(J.ne[k]);
+  }
+  int64_t sum = 0;
+  for (int64_t c : cls) sum += c;
+  if (sum != J.nv[l - 2]) cls.assign(1, J.nv[l - 2]);
+  return cls;
+}
+
+// k_final over several meshes: work units of at most one block each. Corner
+// units: consecutive corners of one mesh, mesh by mesh. Vertex units:
+// consecutive vertices of one class of one mesh; mesh by mesh, the units of
+// all classes are interleaved by their relative position in their class, so
+// that every class advances over the surface together with the corner items
+// (each class is numbered in the order of the original faces).
+void Plan::build_final_units(int l) {
+  LevelPlan& L = levels_[l - 1];
+  L.fin_cunits.clear();
+  L.fin_vunits.clear();
+  for (size_t i = 0; i < L.fin.size(); ++i)
+    for (int64_t c0 = 0; c0 < L.fin[i].nnz; c0 += kBlockItems)
+      L.fin_cunits.push_back(make_int4((int)i, (int)c0, (int)std::min<int64_t>(kBlockItems, L.fin[i].nnz - c0), 0));
+  struct Key {
+    int64_t num, den;  // relative position (2j + 1) / (2 nch) of the chunk
+    int c;
+    int64_t j;
+  };
+  std::vector<Key> keys;
+  for (size_t i = 0; i < L.fin_jobs.size(); ++i) {
+    const std::vector<int64_t> cls = vertex_classes(jobs_[L.fin_jobs[i]], l);
+    std::vector<int64_t> off(cls.size() + 1, 0);
+    for (size_t c = 0; c < cls.size(); ++c) off[c + 1] = off[c] + cls[c];
+    keys.clear();
+    for (size_t c = 0; c < cls.size(); ++c) {
+      const int64_t nch = (cls[c] + kBlockItems - 1) / kBlockItems;
+      for (int64_t j = 0; j < nch; ++j) keys.push_back(Key{2 * j + 1, 2 * nch, (int)c, j});
+    }
+    std::stable_sort(keys.begin(), keys.end(),
+                     [](const Key& a, const Key& b) { return a.num * b.den < b.num * a.den; });
+    for (const Key& k : keys) {
+      const int64_t first = off[k.c] + k.j * kBlockItems;
+      const int64_t count = std::min<int64_t>(kBlockItems, cls[k.c] - k.j * kBlockItems);
+      L.fin_vunits.push_back(make_int4((int)i, (int)first, (int)count, 0));
+    }
+  }
+}
+
 // The last level of the meshes fused at level l (k_final), from the previous
 // level's structure: launched before anything of level l overwrites it (the
 // boundary scan buffer, the alternating M^T buffers).
@@ -804,23 +861,18 @@ void Plan::record_final(int l) {
   V.nv = L.fin_vb.back();
   if (L.fin.size() > 1) {
     V.segs = L.d_fin;
-    V.fcb = L.d_fin_cb;
-    V.fvb = L.d_fin_vb;
+    V.cunits = L.d_fin_cunits;
+    V.vunits = L.d_fin_vunits;
+    V.ncunits = (int32_t)L.fin_cunits.size();
+    V.nvunits = (int32_t)L.fin_vunits.size();
     V.nseg = (int32_t)L.fin.size();
   } else {
     V.one = L.fin[0];
     V.nseg = 1;
     // vertex classes of M: the original vertices, then the face points and
     // edge points of every level before M (leading ones merged past kMaxVSeg)
-    const MeshJob& J = jobs_[L.fin_jobs[0]];
-    std::vector<int64_t> cls{J.nv[0]};
-    for (int k = 0; k + 2 < l; ++k) {
-      cls.push_back(J.nf[k]);
-      cls.push_back(J.ne[k]);
-    }
-    int64_t sum = 0;
-    for (int64_t c : cls) sum += c;
-    if (sum == J.nv[l - 2]) {
+    std::vector<int64_t> cls = vertex_classes(jobs_[L.fin_jobs[0]], l);
+    {
       while (cls.size() > (size_t)kMaxVSeg) {
         cls[1] += cls[0];
         cls.erase(cls.begin());

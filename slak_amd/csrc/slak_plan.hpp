@@ -80,9 +80,10 @@ struct LevelPlan {
   std::vector<int> fin_jobs;
   std::vector<FinalSeg> fin;
   std::vector<int64_t> fin_cb, fin_vb;
+  std::vector<int4> fin_cunits, fin_vunits;  // corner and vertex work units of k_final (several meshes)
   FinalSeg* d_fin = nullptr;
-  int64_t* d_fin_cb = nullptr;
-  int64_t* d_fin_vb = nullptr;
+  int4* d_fin_cunits = nullptr;
+  int4* d_fin_vunits = nullptr;
   SegDesc* d_segs = nullptr;
   int64_t* d_vbase = nullptr;
   int64_t* d_fbase = nullptr;
@@ -176,6 +177,7 @@ class Plan {
   // (k_final) and the level before it writes positions only
   bool fused_final_ = false;
   void record_final(int l);
+  void build_final_units(int l);
   // single mesh: the first level's edge numbering ran during the analysis
   // read-back (the level then launches its face points only)
   bool number_early_ = false;
