@@ -389,9 +389,20 @@ void Plan::prepare() {
   // a single mesh numbers its edges before the read-back: the first upload
   // gives it nnz >= ne as the edge-id bound and a flag buffer of that size
   const bool early = nseg == 1 && jobs_.size() == 1 && jobs_[L1.jobs[0]].known_ne < 0;
+  early_p1_ = nullptr;
+  early_faces_ = false;
   if (early) {
     L1.segs[0].ne = L1.segs[0].nnz;
     L1.bnd = arena_.get_n<int32_t>(L1.nnz);
+    // when the first level feeds another, its positions go to a plan buffer
+    // (sized for nnz >= ne edge points) and the early numbering also forms
+    // the face points
+    const MeshJob& J = jobs_[L1.jobs[0]];
+    if (J.levels >= 2) {
+      early_p1_cap_ = (int64_t)J.in.nverts + (int64_t)J.in.nfaces + (int64_t)J.in.nnz;
+      early_p1_ = arena_.get_n<float4>(early_p1_cap_);
+      L1.segs[0].dst_verts = early_p1_;
+    }
   }
   n_state_ = 1 + 2 * kRowHistBins + (nseg + 1) + nseg;
   upload_tables(0, 1, staging_first_, n_state_);
@@ -429,9 +440,10 @@ void Plan::prepare() {
   bool have_hist = false;
   if (!all_known) {
     std::function<void()> overlap;
-    if (early) overlap = [this, &L1] { launch_number(L1, false); };
+    if (early) overlap = [this, &L1] { launch_number(L1, early_p1_ != nullptr); };
     fail_if(readback(nullptr, 2 * kRowHistBins + nseg + 1, hb.data(), overlap));
     number_early_ = early;
+    early_faces_ = early && early_p1_ != nullptr;
     have_hist = true;
   } else {
     if (!all_trusted) fail_if(readback(nullptr, 0, nullptr));
@@ -607,12 +619,16 @@ void Plan::prepare() {
         nv_next += jobs_[j].nv[l];
         nnz_next += jobs_[j].nnz[l];
       }
-    float4* Un_P = nv_next ? arena_.get_n<float4>(nv_next) : nullptr;
-    int32_t* Un_ids = nnz_next ? arena_.get_n<int32_t>(nnz_next) : nullptr;
     int64_t p1_total = 0, p1_off = 0;  // one block for the positions handed to k_final
     for (int j : L.jobs)
       if (to_final(jobs_[j])) p1_total += jobs_[j].nv[l];
-    float4* p1_block = p1_total ? arena_.get_n<float4>(p1_total) : nullptr;
+    // the first level's positions of a single mesh: the buffer the early
+    // numbering has written the face points into
+    float4* early = l == 1 && early_faces_ && nv_next + p1_total <= early_p1_cap_ ? early_p1_ : nullptr;
+    if (l == 1 && early_faces_ && !early) throw Error(SLAK_ERR_INTERNAL, "first-level positions exceed their buffer");
+    float4* Un_P = nv_next ? (early ? early : arena_.get_n<float4>(nv_next)) : nullptr;
+    int32_t* Un_ids = nnz_next ? arena_.get_n<int32_t>(nnz_next) : nullptr;
+    float4* p1_block = p1_total ? (early ? early : arena_.get_n<float4>(p1_total)) : nullptr;
     int64_t vbn = 0, cbn = 0, ebn = 0;
     for (size_t k = 0; k < L.segs.size(); ++k) {
       MeshJob& J = jobs_[L.jobs[k]];
@@ -908,7 +924,7 @@ void Plan::record_level(int l) {
     // edge numbering of the first level (later levels get theirs in closed form)
     launch_number(L, true);
   }
-  if (l == 1 && number_early_) {
+  if (l == 1 && number_early_ && !early_faces_) {
     ProfScope ps(st_, L.quad ? "facepoints_quad" : "facepoints_poly",
                  4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + (L.quad ? 0.0 : 4.0 * (L.nf + 1)));
     launch_faces(L.quad, true, false, L.ids, L.fptr, L.P, (int32_t)L.nf, L.table(), nullptr, nullptr, st_);

@@ -181,6 +181,11 @@ class Plan {
   // single mesh: the first level's edge numbering ran during the analysis
   // read-back (the level then launches its face points only)
   bool number_early_ = false;
+  // ... and, when the first level feeds another, its face points, into the
+  // first level's position buffer allocated before the read-back
+  bool early_faces_ = false;
+  float4* early_p1_ = nullptr;
+  int64_t early_p1_cap_ = 0;
   void launch_lak_forms();
   std::vector<void*> pooled_outputs_;
 };
