@@ -1561,7 +1561,7 @@ void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st) {
 
 void launch_final(int scheme, const FinalView& V0, hipStream_t st) {
   FinalView V = V0;
-  const bool multi = V.nseg > 1;
+  const bool multi = V.segs != nullptr;
   if (!multi) {
     if (V.one.nnz <= 0 || (V.one.nnz & 3) || (int64_t)V.one.nnz != 4 * (int64_t)V.one.nf)
       throw Error(5, "final level: the mesh two levels back must be quads");  // SLAK_ERR_INTERNAL

@@ -169,7 +169,7 @@ void Plan::upload_tables(int l0, int l1, Staging& staging, int64_t state_ints) {
   // k_final tables of several meshes: FinalSeg[n], corner and vertex item bases
   auto fin_bytes = [&](const LevelPlan& L) -> size_t {
     const size_t n = L.fin.size();
-    return n > 1 ? align(n * sizeof(FinalSeg)) + align(L.fin_cunits.size() * sizeof(int4)) +
+    return final_table(L) ? align(n * sizeof(FinalSeg)) + align(L.fin_cunits.size() * sizeof(int4)) +
                        align(L.fin_vunits.size() * sizeof(int4))
                  : 0;
   };
@@ -212,7 +212,7 @@ void Plan::upload_tables(int l0, int l1, Staging& staging, int64_t state_ints) {
         L.d_mflags = arena_.get_n<int>((int64_t)std::max<size_t>(n, 1));
     }
     const size_t nf = L.fin.size();
-    if (nf > 1) {
+    if (final_table(L)) {
       o = align(o + 3 * (n + 1) * sizeof(int64_t));
       std::memcpy(staging.p + o, L.fin.data(), nf * sizeof(FinalSeg));
       L.d_fin = reinterpret_cast<FinalSeg*>(d + o);
@@ -399,7 +399,7 @@ void Plan::prepare() {
     // the face points
     const MeshJob& J = jobs_[L1.jobs[0]];
     if (J.levels >= 2) {
-      early_p1_cap_ = (int64_t)J.in.nverts + (int64_t)J.in.nfaces + (int64_t)J.in.nnz;
+      early_p1_cap_ = (int64_t)J.in.nverts + (int64_t)J.in.nfaces + (int64_t)J.in.nnz + 8;  // + line padding
       early_p1_ = arena_.get_n<float4>(early_p1_cap_);
       L1.segs[0].dst_verts = early_p1_;
     }
@@ -502,12 +502,17 @@ void Plan::prepare() {
     return nullptr;  // carved from pooled blocks below
   };
   // pooled outputs: one block per kind
+  // each mesh's pooled arrays start on a 128-byte line: the int4 and 64-byte
+  // stores of the final kernels then never straddle lines (a pointer array
+  // of nf + 1 ints would otherwise shift every later mesh by 4 bytes)
+  auto line_v = [](int64_t n) { return (n + 7) & ~(int64_t)7; };     // float4
+  auto line_i = [](int64_t n) { return (n + 31) & ~(int64_t)31; };   // int32
   int64_t tot_v = 0, tot_i = 0, tot_p = 0;
   for (auto& J : jobs_)
     if (J.levels > 0) {
-      tot_v += J.nv[J.levels];
-      tot_i += 4 * J.nnz[J.levels - 1];
-      tot_p += J.nf[J.levels] + 1;
+      tot_v += line_v(J.nv[J.levels]);
+      tot_i += line_i(4 * J.nnz[J.levels - 1]);
+      tot_p += line_i(J.nf[J.levels] + 1);
     }
   float4* pool_v = nullptr;
   int32_t *pool_i = nullptr, *pool_p = nullptr, *pool_a = nullptr;
@@ -539,11 +544,15 @@ void Plan::prepare() {
         J.out_vals = pool_a + oi;
       }
     }
-    ov += nvo;
-    oi += nio;
-    op += npo;
+    ov += line_v(nvo);
+    oi += line_i(nio);
+    op += line_i(npo);
   }
 
+  {
+    const char* e = std::getenv("SLAK_FINAL_TABLE");  // one mesh through the table path too
+    force_final_table_ = e && std::atoi(e) != 0;
+  }
   // k_final computes the last level of every mesh refined two or more levels
   // whose mesh two levels before the output is a quad mesh of a quad level
   // (levels >= 2 are; the first is when every input is a quad mesh)
@@ -621,7 +630,7 @@ void Plan::prepare() {
       }
     int64_t p1_total = 0, p1_off = 0;  // one block for the positions handed to k_final
     for (int j : L.jobs)
-      if (to_final(jobs_[j])) p1_total += jobs_[j].nv[l];
+      if (to_final(jobs_[j])) p1_total += (jobs_[j].nv[l] + 7) & ~(int64_t)7;
     // the first level's positions of a single mesh: the buffer the early
     // numbering has written the face points into
     float4* early = l == 1 && early_faces_ && nv_next + p1_total <= early_p1_cap_ ? early_p1_ : nullptr;
@@ -649,7 +658,7 @@ void Plan::prepare() {
         S.dst_final = 1;
       } else if (to_final(J)) {
         J.p1 = p1_block + p1_off;
-        p1_off += J.nv[l];
+        p1_off += (J.nv[l] + 7) & ~(int64_t)7;  // 128-byte lines
         S.dst_verts = J.p1;
         S.dst_ids = nullptr;
         S.dst_ptr = nullptr;
@@ -773,7 +782,7 @@ void Plan::prepare() {
       L.fin_cb.push_back(L.fin_cb.back() + m.nnz);
       L.fin_vb.push_back(L.fin_vb.back() + m.nv);
     }
-    if (L.fin.size() > 1) build_final_units(l);
+    if (final_table(L)) build_final_units(l);
   }
   upload_tables(0, (int)levels_.size(), staging_all_);
   prepared_ = true;
@@ -875,7 +884,7 @@ void Plan::record_final(int l) {
   V.mpos = M.mpos;
   V.nnz = L.fin_cb.back();
   V.nv = L.fin_vb.back();
-  if (L.fin.size() > 1) {
+  if (final_table(L)) {
     V.segs = L.d_fin;
     V.cunits = L.d_fin_cunits;
     V.vunits = L.d_fin_vunits;

@@ -177,6 +177,11 @@ class Plan {
   // (k_final) and the level before it writes positions only
   bool fused_final_ = false;
   void record_final(int l);
+  // k_final's meshes come from a device table and work units (several meshes)
+  bool force_final_table_ = false;
+  bool final_table(const LevelPlan& L) const {
+    return L.fin.size() > 1 || (force_final_table_ && !L.fin.empty());
+  }
   void build_final_units(int l);
   // single mesh: the first level's edge numbering ran during the analysis
   // read-back (the level then launches its face points only)
