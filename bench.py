@@ -279,12 +279,14 @@ def serve_poisson(n_requests, qps_fraction, scheme, seed=5):
 
 
 # profiler stage -> rocprof kernel name prefix ({s}: scheme template argument)
+# stage -> pattern of its kernel's name (re.match; {s}: 0 LAK, 1 SLAK)
 STAGE_KERNELS = {
-    "refine_vertex_final_quad": "k_final<{s},",
-    "refine_vertex_quad": "k_level<true, {s},",
-    "refine_vertex_poly": "k_level<false, {s},",
-    "facepoints_quad": "k_faces<true, true, false>",
-    "facepoints_poly": "k_faces<false, true, false>",
+    "refine_vertex_final_quad": r"k_final<{s}, \w+, \w+, true>",
+    "refine_vertex_pos_quad": r"k_final<{s}, \w+, \w+, false>",
+    "refine_vertex_quad": r"k_level<true, {s},",
+    "refine_vertex_poly": r"k_level<false, {s},",
+    "facepoints_quad": r"k_faces<true, true, false>",
+    "facepoints_poly": r"k_faces<false, true, false>",
 }
 
 
@@ -357,8 +359,10 @@ def add_traffic(roof, traffic, scheme_name):
     dominant stage's kernels (over the same levels as bytes_per_launch)."""
     if not traffic or roof["kernel"] not in STAGE_KERNELS:
         return roof
-    prefix = STAGE_KERNELS[roof["kernel"]].format(s=0 if scheme_name == "lak" else 1)
-    sel = [v for k, v in traffic["kernels"].items() if k.startswith(prefix)]
+    import re
+
+    pattern = re.compile(STAGE_KERNELS[roof["kernel"]].format(s=0 if scheme_name == "lak" else 1))
+    sel = [v for k, v in traffic["kernels"].items() if pattern.match(k)]
     n = sum(v["dispatches"] for v in sel)
     if not n:
         return roof

@@ -59,7 +59,8 @@ void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st);
 // refine (per corner) and vertex (per vertex) items of a level in one launch
 void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st);
 // final level of a single mesh from the quad mesh two levels back
-void launch_final(int scheme, const FinalView& V, hipStream_t st);
+// ids: also the output quads and LAK form (false: positions only)
+void launch_final(int scheme, const FinalView& V, bool ids, hipStream_t st);
 // LAK form (ptr, vals) of every segment of a level with dst_ptr set; nnz =
 // the level's union corners (= refined faces)
 void launch_lak_form(const SegTable& seg, int64_t nnz, hipStream_t st);

@@ -827,7 +827,7 @@ constexpr int kStageStride = 5;  // int4 per lane in the staging area (4 used; 5
 
 // Corner t (mesh-local) of M in mesh S; its 16 output ids go to the block's
 // staging area.
-template <int SCHEME, bool BND>
+template <int SCHEME, bool BND, bool IDS>
 __device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg& S, int32_t t,
                                              int4* __restrict__ stage) {
   // every lane of a quad runs (nnz = 4 nf, meshes and blocks hold whole quads)
@@ -896,7 +896,9 @@ __device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg&
 
   // M1 edge ids of the corners 4t .. 4t+3 (as the refine items number M1)
   int32_t ex, ey, ez, ew;
-  if (SCHEME == kLAK) {
+  if (!IDS) {
+    ex = ey = ez = ew = 0;
+  } else if (SCHEME == kLAK) {
     const int32_t base1 = 4 * e1 - bl1, base0 = 4 * e0 - bl0;
     ex = base1 + (v > w ? 1 : 0);
     ey = base1 + 2 + ((s1 >= 0 && F > g1) ? 1 : 0);
@@ -909,7 +911,7 @@ __device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg&
     ew = s0 < 0 ? 2 * nnz + bl0 : mp_s0;
   }
   // the four output quads of M1 face t
-  {
+  if (IDS) {
     const int32_t off = S.id_off, eb2 = epb2 + off;
     const int32_t x0 = v + off, x1 = ep0 + e1 + off, x2 = nv + F + off, x3 = ep0 + e0 + off, fid = nv1 + t + off;
     // into the block's staging area: the block writes them out as 1 KB runs
@@ -1082,7 +1084,7 @@ __device__ __forceinline__ void final_vertex(const FinalView& V, const FinalSeg&
 // positions they gather are fetched once for both. A corner block stages its
 // 16 KB of output quads in LDS and stores them with consecutive lanes on
 // consecutive 16-byte pieces.
-template <int SCHEME, bool BND, bool MULTI>
+template <int SCHEME, bool BND, bool MULTI, bool IDS>
 __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int32_t nbv) {
   const int64_t b = block_index<kRemapFinal>();
   const int64_t T = (int64_t)nbc + nbv;
@@ -1110,7 +1112,6 @@ __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int3
   } else {
     // one mesh per corner block: several meshes take it from the block's
     // corner unit (block-uniform, like the single mesh's descriptor)
-    __shared__ int4 stage[kStageStride * kBlock];
     const int64_t cblk = b - before;
     FinalSeg S;
     int32_t t0, cnt;
@@ -1125,7 +1126,12 @@ __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int3
       cnt = (int32_t)min((int64_t)kBlock, V.nnz - t0);
     }
     const int32_t t = t0 + (int32_t)threadIdx.x;
-    if ((int)threadIdx.x < cnt) final_corner<SCHEME, BND>(V, S, t, stage);
+    if (!IDS) {  // positions only
+      if ((int)threadIdx.x < cnt) final_corner<SCHEME, BND, false>(V, S, t, nullptr);
+      return;
+    }
+    __shared__ int4 stage[kStageStride * kBlock];
+    if ((int)threadIdx.x < cnt) final_corner<SCHEME, BND, true>(V, S, t, stage);
     __syncthreads();  // the whole block: corner blocks are uniform
     const int nq = 4 * cnt;
     int32_t* __restrict__ dst = S.out_ids + 16 * (int64_t)t0;
@@ -1559,7 +1565,7 @@ void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st) {
   }
 }
 
-void launch_final(int scheme, const FinalView& V0, hipStream_t st) {
+void launch_final(int scheme, const FinalView& V0, bool ids, hipStream_t st) {
   FinalView V = V0;
   const bool multi = V.segs != nullptr;
   if (!multi) {
@@ -1596,7 +1602,13 @@ void launch_final(int scheme, const FinalView& V0, hipStream_t st) {
   if ((nbc + nbv) * kBlock > 0xffffffffll) throw Error(1, "level too large for one launch");
   const unsigned g = (unsigned)(nbc + nbv);
   const bool bnd = V.blk != nullptr;
-#define SLAK_FINAL(S, B, M) k_final<S, B, M><<<g, kBlock, 0, st>>>(V, (int32_t)nbc, (int32_t)nbv)
+#define SLAK_FINAL(S, B, M)                                                    \
+  do {                                                                         \
+    if (ids)                                                                   \
+      k_final<S, B, M, true><<<g, kBlock, 0, st>>>(V, (int32_t)nbc, (int32_t)nbv);  \
+    else                                                                       \
+      k_final<S, B, M, false><<<g, kBlock, 0, st>>>(V, (int32_t)nbc, (int32_t)nbv); \
+  } while (0)
   if (scheme == kLAK) {
     if (multi) {
       if (bnd) SLAK_FINAL(kLAK, true, true);

@@ -167,12 +167,12 @@ void Plan::upload_tables(int l0, int l1, Staging& staging, int64_t state_ints) {
   size_t total = 0;
   std::vector<size_t> offs;
   // k_final tables of several meshes: FinalSeg[n], corner and vertex item bases
-  auto fin_bytes = [&](const LevelPlan& L) -> size_t {
-    const size_t n = L.fin.size();
-    return final_table(L) ? align(n * sizeof(FinalSeg)) + align(L.fin_cunits.size() * sizeof(int4)) +
-                       align(L.fin_vunits.size() * sizeof(int4))
-                 : 0;
+  auto table_bytes = [&](const FinalTable& T) -> size_t {
+    return final_table(T) ? align(T.segs.size() * sizeof(FinalSeg)) + align(T.cunits.size() * sizeof(int4)) +
+                                align(T.vunits.size() * sizeof(int4))
+                          : 0;
   };
+  auto fin_bytes = [&](const LevelPlan& L) -> size_t { return table_bytes(L.fin) + table_bytes(L.pos); };
   for (int l = l0; l < l1; ++l) {
     const size_t n = levels_[l].segs.size();
     offs.push_back(total);
@@ -211,17 +211,18 @@ void Plan::upload_tables(int l0, int l1, Staging& staging, int64_t state_ints) {
       else
         L.d_mflags = arena_.get_n<int>((int64_t)std::max<size_t>(n, 1));
     }
-    const size_t nf = L.fin.size();
-    if (final_table(L)) {
-      o = align(o + 3 * (n + 1) * sizeof(int64_t));
-      std::memcpy(staging.p + o, L.fin.data(), nf * sizeof(FinalSeg));
-      L.d_fin = reinterpret_cast<FinalSeg*>(d + o);
-      o = align(o + nf * sizeof(FinalSeg));
-      std::memcpy(staging.p + o, L.fin_cunits.data(), L.fin_cunits.size() * sizeof(int4));
-      L.d_fin_cunits = reinterpret_cast<int4*>(d + o);
-      o = align(o + L.fin_cunits.size() * sizeof(int4));
-      std::memcpy(staging.p + o, L.fin_vunits.data(), L.fin_vunits.size() * sizeof(int4));
-      L.d_fin_vunits = reinterpret_cast<int4*>(d + o);
+    o = align(o + 3 * (n + 1) * sizeof(int64_t));
+    for (FinalTable* T : {&L.fin, &L.pos}) {
+      if (!final_table(*T)) continue;
+      std::memcpy(staging.p + o, T->segs.data(), T->segs.size() * sizeof(FinalSeg));
+      T->d_segs = reinterpret_cast<FinalSeg*>(d + o);
+      o = align(o + T->segs.size() * sizeof(FinalSeg));
+      std::memcpy(staging.p + o, T->cunits.data(), T->cunits.size() * sizeof(int4));
+      T->d_cunits = reinterpret_cast<int4*>(d + o);
+      o = align(o + T->cunits.size() * sizeof(int4));
+      std::memcpy(staging.p + o, T->vunits.data(), T->vunits.size() * sizeof(int4));
+      T->d_vunits = reinterpret_cast<int4*>(d + o);
+      o = align(o + T->vunits.size() * sizeof(int4));
     }
   }
   SLAK_HIP_CHECK(hipMemcpyAsync(d, staging.p, total, hipMemcpyHostToDevice, st_));
@@ -581,12 +582,14 @@ void Plan::prepare() {
       L.fbase.clear();
       L.cbase.clear();
       int64_t vb = 0, fb = 0, cb = 0, eb = 0;
-      L.fin_jobs.clear();
+      L.fin = FinalTable{};
+      L.pos = FinalTable{};
+      L.pos_fused = false;
       for (int j : prev.jobs) {
         MeshJob& J = jobs_[j];
         if (J.levels < l) continue;
         if (J.fused && J.levels == l) {  // formed by k_final, outside the union
-          L.fin_jobs.push_back(j);
+          L.fin.jobs.push_back(j);
           continue;
         }
         SegDesc S{};
@@ -751,38 +754,35 @@ void Plan::prepare() {
   // k_final tables: per fused mesh, its segment of M (the level before's
   // union), the positions of M1 and its outputs
   std::vector<int> seg_in_m(jobs_.size(), -1);  // job -> its segment of M
+  const char* pe = std::getenv("SLAK_FUSED_POS");
+  const bool allow_pos = !pe || std::atoi(pe) != 0;
   for (int l = 2; l <= Lmax; ++l) {
     LevelPlan& L = levels_[l - 1];
     const LevelPlan& M = levels_[l - 2];
-    L.fin.clear();
-    L.fin_cb.assign(1, 0);
-    L.fin_vb.assign(1, 0);
-    if (L.fin_jobs.empty()) continue;
     for (size_t k = 0; k < M.jobs.size(); ++k) seg_in_m[M.jobs[k]] = (int)k;
-    for (int j : L.fin_jobs) {
-      const MeshJob& J = jobs_[j];
+    // meshes whose last level is l
+    for (int j : L.fin.jobs) {
       const int k = seg_in_m[j];
-      if (k < 0 || !J.p1) throw Error(SLAK_ERR_INTERNAL, "fused level without its input");
-      const SegDesc& m = M.segs[k];
-      FinalSeg F{};
-      F.cb = m.cb;
-      F.vb = m.vb;
-      F.eb = m.eb;
-      F.nv = m.nv;
-      F.nf = m.nf;
-      F.ne = (int32_t)m.ne;
-      F.nnz = (int32_t)m.nnz;
-      F.id_off = 0;
-      F.P1 = J.p1;
-      F.out_verts = J.out_verts;
-      F.out_ids = J.out_ids;
-      F.out_ptr = J.out_ptr;
-      F.out_vals = J.out_vals;
-      L.fin.push_back(F);
-      L.fin_cb.push_back(L.fin_cb.back() + m.nnz);
-      L.fin_vb.push_back(L.fin_vb.back() + m.nv);
+      if (k < 0 || !jobs_[j].p1) throw Error(SLAK_ERR_INTERNAL, "fused level without its input");
+      add_final_seg(L.fin, M.segs[k], jobs_[j].p1, jobs_[j], true);
     }
-    if (final_table(L)) build_final_units(l);
+    if (final_table(L.fin)) build_final_units(l, L.fin);
+    // a level whose every segment hands its positions to k_final at the next
+    // level, with a quad mesh two levels back: its positions from that mesh's
+    // structure too (positions only)
+    bool pos = allow_pos && !L.segs.empty() && (l >= 3 || levels_[0].quad);
+    for (int j : L.jobs) pos = pos && jobs_[j].fused && jobs_[j].levels == l + 1 && jobs_[j].p1;
+    if (pos) {
+      L.pos_fused = true;
+      for (size_t k = 0; k < L.segs.size(); ++k) {
+        const int j = L.jobs[k];
+        const int km = seg_in_m[j];
+        if (km < 0) throw Error(SLAK_ERR_INTERNAL, "fused level without its input");
+        L.pos.jobs.push_back(j);
+        add_final_seg(L.pos, M.segs[km], L.P + L.segs[k].vb, jobs_[j], false);
+      }
+      if (final_table(L.pos)) build_final_units(l, L.pos);
+    }
   }
   upload_tables(0, (int)levels_.size(), staging_all_);
   prepared_ = true;
@@ -836,21 +836,20 @@ static std::vector<int64_t> vertex_classes(const MeshJob& J, int l) {
 // all classes are interleaved by their relative position in their class, so
 // that every class advances over the surface together with the corner items
 // (each class is numbered in the order of the original faces).
-void Plan::build_final_units(int l) {
-  LevelPlan& L = levels_[l - 1];
-  L.fin_cunits.clear();
-  L.fin_vunits.clear();
-  for (size_t i = 0; i < L.fin.size(); ++i)
-    for (int64_t c0 = 0; c0 < L.fin[i].nnz; c0 += kBlockItems)
-      L.fin_cunits.push_back(make_int4((int)i, (int)c0, (int)std::min<int64_t>(kBlockItems, L.fin[i].nnz - c0), 0));
+void Plan::build_final_units(int l, FinalTable& T) {
+  T.cunits.clear();
+  T.vunits.clear();
+  for (size_t i = 0; i < T.segs.size(); ++i)
+    for (int64_t c0 = 0; c0 < T.segs[i].nnz; c0 += kBlockItems)
+      T.cunits.push_back(make_int4((int)i, (int)c0, (int)std::min<int64_t>(kBlockItems, T.segs[i].nnz - c0), 0));
   struct Key {
     int64_t num, den;  // relative position (2j + 1) / (2 nch) of the chunk
     int c;
     int64_t j;
   };
   std::vector<Key> keys;
-  for (size_t i = 0; i < L.fin_jobs.size(); ++i) {
-    const std::vector<int64_t> cls = vertex_classes(jobs_[L.fin_jobs[i]], l);
+  for (size_t i = 0; i < T.jobs.size(); ++i) {
+    const std::vector<int64_t> cls = vertex_classes(jobs_[T.jobs[i]], l);
     std::vector<int64_t> off(cls.size() + 1, 0);
     for (size_t c = 0; c < cls.size(); ++c) off[c + 1] = off[c] + cls[c];
     keys.clear();
@@ -863,17 +862,47 @@ void Plan::build_final_units(int l) {
     for (const Key& k : keys) {
       const int64_t first = off[k.c] + k.j * kBlockItems;
       const int64_t count = std::min<int64_t>(kBlockItems, cls[k.c] - k.j * kBlockItems);
-      L.fin_vunits.push_back(make_int4((int)i, (int)first, (int)count, 0));
+      T.vunits.push_back(make_int4((int)i, (int)first, (int)count, 0));
     }
   }
 }
 
-// The last level of the meshes fused at level l (k_final), from the previous
-// level's structure: launched before anything of level l overwrites it (the
-// boundary scan buffer, the alternating M^T buffers).
-void Plan::record_final(int l) {
-  LevelPlan& L = levels_[l - 1];
-  if (L.fin.empty()) return;
+// One mesh of a k_final table: m is its segment of M, P1 the positions of
+// M1. The output level writes the job's outputs (quads, LAK form,
+// positions); the level before it writes the positions handed to k_final.
+void Plan::add_final_seg(FinalTable& T, const SegDesc& m, const float4* P1, const MeshJob& J, bool output_level) {
+  FinalSeg F{};
+  F.cb = m.cb;
+  F.vb = m.vb;
+  F.eb = m.eb;
+  F.nv = m.nv;
+  F.nf = m.nf;
+  F.ne = (int32_t)m.ne;
+  F.nnz = (int32_t)m.nnz;
+  F.id_off = 0;
+  F.P1 = P1;
+  F.out_verts = output_level ? J.out_verts : J.p1;
+  if (output_level) {
+    F.out_ids = J.out_ids;
+    F.out_ptr = J.out_ptr;
+    F.out_vals = J.out_vals;
+  }
+  if (T.segs.empty()) {
+    T.cb.assign(1, 0);
+    T.vb.assign(1, 0);
+  }
+  T.segs.push_back(F);
+  T.cb.push_back(T.cb.back() + m.nnz);
+  T.vb.push_back(T.vb.back() + m.nv);
+}
+
+// k_final over the meshes of table T at level l, from the structure of their
+// mesh two levels back (the previous level's union): launched before anything
+// of level l overwrites that structure (the boundary scan buffer, the
+// alternating M^T buffers). ids: the output level (quads, LAK form,
+// positions); otherwise positions only.
+void Plan::launch_final_table(int l, FinalTable& T, bool ids, const char* stage) {
+  if (T.empty()) return;
   const LevelPlan& M = levels_[l - 2];
   FinalView V{};
   V.ids = M.ids;
@@ -882,50 +911,53 @@ void Plan::record_final(int l) {
   V.trow = M.trow;
   V.blk = M.blk;
   V.mpos = M.mpos;
-  V.nnz = L.fin_cb.back();
-  V.nv = L.fin_vb.back();
-  if (final_table(L)) {
-    V.segs = L.d_fin;
-    V.cunits = L.d_fin_cunits;
-    V.vunits = L.d_fin_vunits;
-    V.ncunits = (int32_t)L.fin_cunits.size();
-    V.nvunits = (int32_t)L.fin_vunits.size();
-    V.nseg = (int32_t)L.fin.size();
+  V.nnz = T.cb.back();
+  V.nv = T.vb.back();
+  if (final_table(T)) {
+    V.segs = T.d_segs;
+    V.cunits = T.d_cunits;
+    V.vunits = T.d_vunits;
+    V.ncunits = (int32_t)T.cunits.size();
+    V.nvunits = (int32_t)T.vunits.size();
+    V.nseg = (int32_t)T.segs.size();
   } else {
-    V.one = L.fin[0];
+    V.one = T.segs[0];
     V.nseg = 1;
     // vertex classes of M: the original vertices, then the face points and
     // edge points of every level before M (leading ones merged past kMaxVSeg)
-    std::vector<int64_t> cls = vertex_classes(jobs_[L.fin_jobs[0]], l);
-    {
-      while (cls.size() > (size_t)kMaxVSeg) {
-        cls[1] += cls[0];
-        cls.erase(cls.begin());
-      }
-      V.nvseg = (int32_t)cls.size();
-      V.vseg[0] = 0;
-      for (size_t i = 0; i < cls.size(); ++i) V.vseg[i + 1] = V.vseg[i] + (int32_t)cls[i];
+    std::vector<int64_t> cls = vertex_classes(jobs_[T.jobs[0]], l);
+    while (cls.size() > (size_t)kMaxVSeg) {
+      cls[1] += cls[0];
+      cls.erase(cls.begin());
     }
+    V.nvseg = (int32_t)cls.size();
+    V.vseg[0] = 0;
+    for (size_t i = 0; i < cls.size(); ++i) V.vseg[i + 1] = V.vseg[i] + (int32_t)cls[i];
   }
   // algorithmic bytes: reads the ids, (edge, twin) pairs, M^T rows and
   // offsets of M (boundary scan, SLAK positions) and the positions of M1;
-  // writes the output quads and every output position (and the LAK form)
+  // writes every output position and, for the output level, the quads (and
+  // the LAK form)
   double reads = 0, writes = 0;
-  for (int j : L.fin_jobs) {
+  for (int j : T.jobs) {
     const MeshJob& J = jobs_[j];
     const double mnnz = (double)J.nnz[l - 2], mnv = (double)J.nv[l - 2], mne = (double)J.ne[l - 2];
     const double nnz1 = (double)J.nnz[l - 1], nv1 = (double)J.nv[l - 1], nf1 = (double)J.nf[l - 1];
     const double ne1 = (double)J.ne[l - 1];
     reads += 20.0 * mnnz + 4.0 * mnv + (M.blk ? 4.0 * mne : 0.0) + (scheme_ == kSLAK ? 4.0 * mnnz : 0.0) + 16.0 * nv1;
-    writes += 16.0 * nnz1 + 16.0 * nv1 + 16.0 * nf1 + 16.0 * ne1 + (J.out_ptr ? 4.0 * (nnz1 + 1) + 16.0 * nnz1 : 0.0);
+    writes += 16.0 * nv1 + 16.0 * nf1 + 16.0 * ne1;
+    if (ids) writes += 16.0 * nnz1 + (J.out_ptr ? 4.0 * (nnz1 + 1) + 16.0 * nnz1 : 0.0);
   }
-  ProfScope ps(st_, "refine_vertex_final_quad", reads + writes);
-  launch_final(scheme_, V, st_);
+  ProfScope ps(st_, stage, reads + writes);
+  launch_final(scheme_, V, ids, st_);
 }
 
 void Plan::record_level(int l) {
   LevelPlan& L = levels_[l - 1];
-  if (l > 1) record_final(l);
+  if (l > 1) {
+    launch_final_table(l, L.fin, true, "refine_vertex_final_quad");
+    if (L.pos_fused) launch_final_table(l, L.pos, false, "refine_vertex_pos_quad");
+  }
   if (L.segs.empty()) return;  // every mesh of the level was fused
   const double poly_b = L.quad ? 0.0 : 4.0 * (L.nf + 1) + 4.0 * L.nnz;
   const double ne = (double)L.ne;
@@ -942,6 +974,7 @@ void Plan::record_level(int l) {
     ProfScope ps(st_, "scan", 8.0 * ne + 4);
     launch_scan(L.bnd, L.blk, L.ne, d_partial_, d_flags_, st_);
   }
+  if (L.pos_fused) return;  // its positions came from k_final; the scan above serves the next level's
   if (l > 1) {
     ProfScope ps(st_, L.quad ? "facepoints_quad" : "facepoints_poly",
                  4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + (L.quad ? 0.0 : 4.0 * (L.nf + 1)));

@@ -42,6 +42,20 @@ struct MeshJob {
   float4* p1 = nullptr;
 };
 
+// Meshes whose level is formed by k_final from the structure of their mesh M
+// two levels before it: per mesh its segment of M, the positions of M1 and
+// the outputs; item bases; work units of the table path (several meshes).
+struct FinalTable {
+  std::vector<int> jobs;
+  std::vector<FinalSeg> segs;
+  std::vector<int64_t> cb, vb;  // corner and vertex item bases (n + 1)
+  std::vector<int4> cunits, vunits;
+  FinalSeg* d_segs = nullptr;
+  int4* d_cunits = nullptr;
+  int4* d_vunits = nullptr;
+  bool empty() const { return segs.empty(); }
+};
+
 struct LevelPlan {
   bool quad = true;
   int64_t nv = 0, nf = 0, nnz = 0;  // union sizes
@@ -74,16 +88,14 @@ struct LevelPlan {
   std::vector<SegDesc> segs;
   std::vector<int64_t> vbase, fbase, cbase;
   bool lak_form = false;  // some mesh finishes here with the LAK form (ptr / vals)
-  // meshes whose last level is this one, formed by k_final from the previous
-  // level's structure (they are not segments of this level's union):
-  // per-mesh table and item bases (corners and vertices of M, nfin + 1)
-  std::vector<int> fin_jobs;
-  std::vector<FinalSeg> fin;
-  std::vector<int64_t> fin_cb, fin_vb;
-  std::vector<int4> fin_cunits, fin_vunits;  // corner and vertex work units of k_final (several meshes)
-  FinalSeg* d_fin = nullptr;
-  int4* d_fin_cunits = nullptr;
-  int4* d_fin_vunits = nullptr;
+  // meshes whose last level is this one, formed by k_final (output quads,
+  // LAK form, positions); they are not segments of this level's union
+  FinalTable fin;
+  // every segment of this level hands its positions to k_final at the next
+  // level: they are formed here by k_final's positions-only variant, and
+  // this level launches no face points and no refine or vertex items
+  bool pos_fused = false;
+  FinalTable pos;
   SegDesc* d_segs = nullptr;
   int64_t* d_vbase = nullptr;
   int64_t* d_fbase = nullptr;
@@ -176,13 +188,12 @@ class Plan {
   // its output: its last level is computed from that mesh's structure
   // (k_final) and the level before it writes positions only
   bool fused_final_ = false;
-  void record_final(int l);
   // k_final's meshes come from a device table and work units (several meshes)
   bool force_final_table_ = false;
-  bool final_table(const LevelPlan& L) const {
-    return L.fin.size() > 1 || (force_final_table_ && !L.fin.empty());
-  }
-  void build_final_units(int l);
+  bool final_table(const FinalTable& T) const { return T.segs.size() > 1 || (force_final_table_ && !T.empty()); }
+  void add_final_seg(FinalTable& T, const SegDesc& m, const float4* P1, const MeshJob& J, bool output_level);
+  void launch_final_table(int l, FinalTable& T, bool ids, const char* stage);
+  void build_final_units(int l, FinalTable& T);
   // single mesh: the first level's edge numbering ran during the analysis
   // read-back (the level then launches its face points only)
   bool number_early_ = false;
