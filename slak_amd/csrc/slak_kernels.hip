@@ -956,10 +956,11 @@ __device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg&
     i_fF = nnz + tn;
     i_fG = nnz + n1;
   }
-  // LAK: one 64-byte group, cached; SLAK: four scattered points, non-temporal
+  // LAK: one 64-byte group, cached; SLAK: four scattered points,
+  // non-temporal at the output level (cached when the next kernel reads them)
   float4* __restrict__ eo = out + epb2;
   const float4 q_fG = f4mul(f4add(f4add(f4add(PG, Pe), fpsn), fps), 0.25f);
-  if (SCHEME == kLAK) {
+  if (SCHEME == kLAK || !IDS) {
     eo[i_vh] = q_vh;
     eo[i_wh] = q_wh;
     eo[i_fF] = q_fF;
