@@ -281,3 +281,31 @@ def test_row_capacity_variants_bitwise_equal(monkeypatch, cap):
     got, _ = subdivide_host(m, 3, LAK)
     np.testing.assert_array_equal(got.ids, ref.ids)
     np.testing.assert_array_equal(got.verts, ref.verts)
+
+
+@pytest.mark.parametrize("scheme", [LAK, SLAK], ids=["LAK", "SLAK"])
+@pytest.mark.parametrize("case", ["mixed_4", "grid_3", "torus_2"])
+@pytest.mark.parametrize("env", [{"SLAK_FUSED_POS": "0"}, {"SLAK_FINAL_TABLE": "1"}, {"SLAK_FUSED_FINAL": "0"}],
+                         ids=["no_pos", "table", "level_by_level"])
+def test_closed_form_levels_equal_level_by_level(scheme, case, env, monkeypatch):
+    """The last level (k_final) and the level before it (positions-only
+    k_final) formed from the quad mesh two levels back give the same bits as
+    the level-by-level chain, through the one-mesh and the table path:
+    polygonal input (closed forms from level 1), an open quad grid at 3 levels
+    (the level before the last from the input's own structure, boundary) and a
+    quad torus at 2 levels (last level from the input)."""
+    mesh, levels = {
+        "mixed_4": (lambda: g.torus_mixed(20, 12, 0.3, pentagons=True, seed=9), 4),
+        "grid_3": (lambda: g.open_grid(9, 7, sigma=0.01, seed=3), 3),
+        "torus_2": (lambda: g.torus(12, 8, seed=5), 2),
+    }[case]
+    h = mesh()
+    base, _ = subdivide_host(h, levels, scheme)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    other, _ = subdivide_host(h, levels, scheme)
+    np.testing.assert_array_equal(other.ids, base.ids)
+    np.testing.assert_array_equal(other.verts, base.verts)
+    if scheme == LAK:
+        np.testing.assert_array_equal(other.ptr, base.ptr)
+        np.testing.assert_array_equal(other.vals, base.vals)
