@@ -936,6 +936,17 @@ __device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg&
     s = f4add(s, pq3);
     out[nv + F] = f4div(f4add(f4div(s, 4.0f), f4mul(PF, 2.0f)), 4.0f);
   }
+  if (SCHEME == kSLAK) {
+    // the M1 edge from ep(e1) to fp(F), SLAK slot nnz + next(t): every lane
+    // writes its own, so a wave stores 1 KB of consecutive slots (the owner
+    // of e1 would form the same sum, in the same order, for both faces)
+    const float4 q = f4mul(f4add(f4add(f4add(PF, Pe), fptn), fpt), 0.25f);
+    float4* const d = out + epb2 + nnz + tn;
+    if (IDS)
+      store_nt(d, q);
+    else
+      *d = q;
+  }
   if (!own) return;
   // edge points of the M1 edges around ep(e1), by lower end point then the
   // faces of the half-edges lower -> upper and upper -> lower
@@ -956,20 +967,22 @@ __device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg&
     i_fF = nnz + tn;
     i_fG = nnz + n1;
   }
-  // LAK: one 64-byte group, cached; SLAK: four scattered points,
-  // non-temporal at the output level (cached when the next kernel reads them)
+  // LAK: one 64-byte group, cached; SLAK: the two halves of e1 (the edges
+  // to the face points are written above), non-temporal at the output level
+  // (cached when the next kernel reads them)
   float4* __restrict__ eo = out + epb2;
   const float4 q_fG = f4mul(f4add(f4add(f4add(PG, Pe), fpsn), fps), 0.25f);
-  if (SCHEME == kLAK || !IDS) {
+  if (SCHEME == kLAK) {
     eo[i_vh] = q_vh;
     eo[i_wh] = q_wh;
     eo[i_fF] = q_fF;
     if (inner) eo[i_fG] = q_fG;
+  } else if (!IDS) {
+    eo[i_vh] = q_vh;
+    eo[i_wh] = q_wh;
   } else {
     store_nt(eo + i_vh, q_vh);
     store_nt(eo + i_wh, q_wh);
-    store_nt(eo + i_fF, q_fF);
-    if (inner) store_nt(eo + i_fG, q_fG);
   }
   // vertex point of ep(e1)
   float4 q;
