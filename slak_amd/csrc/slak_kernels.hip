@@ -774,10 +774,11 @@ __global__ void __launch_bounds__(kBlock) k_level(RefineView R, int32_t nbr, int
 }
 
 // ---------------------------------------------------------------------------
-// Final level of a single mesh in closed form from the quad mesh M two levels
-// before the output. M1 (the refinement of M, whose positions P1 are in hand;
-// its ids and structure are never written) has corners 4t + k for the
-// corners t of M:
+// A level in closed form from the quad mesh M two levels before it (one mesh,
+// or several through a device table and work units): the output level (IDS:
+// quads, LAK form, positions) or the level before it (positions only). M1
+// (the refinement of M, whose positions P1 are in hand; its ids and
+// structure are never written) has corners 4t + k for the corners t of M:
 //   quad t of M1   (v, ep(e1), fp(F), ep(e0))  in M1 vertex ids
 //   M1 edge ids    corner 4t + k leaves along E1[k] = (ex, ey, ez, ew) and
 //                  enters along E0[k] = (ew, ex, ey, ez), the ids the refine
@@ -785,10 +786,12 @@ __global__ void __launch_bounds__(kBlock) k_level(RefineView R, int32_t nbr, int
 //   output quad    of M1 corner 4t + k: (vertex, ep'(E1[k]), fp'(t), ep'(E0[k]))
 // One lane per corner t of M computes the face point of M1 face t from its
 // four corners (the fourth comes from the previous lane) and its four output
-// quads (64 contiguous bytes, written out by the block through LDS). The owner of edge e1 of M (t < twin, or no twin) writes the edge
-// points of the three or four M1 edges around ep(e1) (both halves of e1 and
-// the edges to fp(F) and fp(G)), contiguous for LAK, and the vertex point of
-// ep(e1), from the same nine positions. Lane 0 of each quad writes the vertex
+// quads (64 contiguous bytes, written out by the block through LDS). The
+// owner of edge e1 of M (t < twin, or no twin) writes the edge points of the
+// three or four M1 edges around ep(e1) (both halves of e1 and the edges to
+// fp(F) and fp(G); for SLAK each lane writes its own edge to fp(F) instead),
+// contiguous for LAK, and the vertex point of ep(e1), from the same nine
+// positions. Lane 0 of each quad writes the vertex
 // point of fp(F) from its four lanes. Vertex items remain only for the
 // vertices of M. Every sum adds the same terms in the same order as the
 // refine and vertex items of M1 would: the results are the same bits.
