@@ -125,7 +125,11 @@ class Arena {
   T* get_n(int64_t n) {
     return static_cast<T*>(get(sizeof(T) * (size_t)(n > 0 ? n : 1)));
   }
-  void keep(void* p);  // ownership moves to the caller
+  // Later get() calls are carved (256-byte aligned) from one pool block of
+  // `bytes` while it lasts: one pool round trip instead of dozens while a
+  // plan is set up. Blocks carved this way cannot be kept.
+  void reserve(size_t bytes);
+  void keep(void* p);  // ownership moves to the caller (blocks from get() without a reserve)
   void release_all();
   bool empty() const { return blocks_.empty(); }
   size_t bytes() const { return bytes_; }
@@ -134,6 +138,8 @@ class Arena {
   Pool& pool_;
   std::vector<void*> blocks_;
   size_t bytes_ = 0;
+  char* bump_ = nullptr;
+  size_t bump_left_ = 0;
 };
 
 // ---------------------------------------------------------------------------

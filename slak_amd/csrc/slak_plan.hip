@@ -496,6 +496,36 @@ void Plan::prepare() {
     }
   }
 
+  // one arena block for the level buffers that follow (positions and ids
+  // handed on, the positions for k_final, the alternating structure, the
+  // boundary flags and their scan, the tables), estimated from the
+  // closed-form sizes; anything beyond it comes from the pool as before
+  if (Lmax >= 1) {
+    auto line = [](int64_t n, size_t elem) { return ((size_t)std::max<int64_t>(n, 1) * elem + 255) & ~(size_t)255; };
+    size_t est = 0;
+    int64_t mnv = 0, mnnz = 0, mne = 0;
+    for (int l = 1; l <= Lmax; ++l) {
+      int64_t nv = 0, nnz = 0, ne = 0, pad = 0;
+      for (int j : levels_[0].jobs) {
+        const MeshJob& J = jobs_[j];
+        if (J.levels > l) {
+          nv += J.nv[l];
+          nnz += J.nnz[l];
+          pad += 8;
+        }
+        if (J.levels >= l) ne += J.ne[l - 1];
+      }
+      est += line(nv + pad, 16) + 256 + line(nnz, 4);
+      mnv = std::max(mnv, nv);
+      mnnz = std::max(mnnz, nnz);
+      mne = std::max(mne, ne);
+    }
+    est += 2 * (line(mnv + 1, 4) + 2 * line(mnnz, 8) + line(mnnz, 4));
+    est += 2 * line(mne + 1, 4) + line(scan_partials_len(mne), 8);
+    est += (size_t)1 << 20;
+    arena_.reserve(est);
+  }
+
   // 4. level unions, destinations and outputs
   const bool lak_scheme = scheme_ == kLAK;
   auto out_alloc = [&](size_t bytes) -> void* {

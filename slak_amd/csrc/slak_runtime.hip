@@ -138,10 +138,27 @@ Pool& pool() {
 }
 
 void* Arena::get(size_t bytes) {
+  const size_t b = (bytes + kAlign - 1) / kAlign * kAlign;
+  if (b > 0 && b <= bump_left_) {
+    void* p = bump_;
+    bump_ += b;
+    bump_left_ -= b;
+    return p;
+  }
   void* p = pool_.alloc(bytes);
   blocks_.push_back(p);
   bytes_ += bytes;
   return p;
+}
+
+void Arena::reserve(size_t bytes) {
+  if (bytes == 0) return;
+  bytes = (bytes + kAlign - 1) / kAlign * kAlign;
+  void* p = pool_.alloc(bytes);
+  blocks_.push_back(p);
+  bytes_ += bytes;
+  bump_ = static_cast<char*>(p);
+  bump_left_ = bytes;
 }
 
 void Arena::keep(void* p) {
@@ -152,6 +169,8 @@ void Arena::keep(void* p) {
 void Arena::release_all() {
   for (void* p : blocks_) pool_.release(p);
   blocks_.clear();
+  bump_ = nullptr;
+  bump_left_ = 0;
 }
 
 // ---------------------------------------------------------------------------
