@@ -288,6 +288,19 @@ void Plan::plan_first_level() {
   check_i32(L.nnz, "corners of the first level");
   check_i32(4 * L.nnz, "refined corners of the first level");
 
+  // one arena block for the first level's buffers and the analysis state
+  // that follows (the table upload, the scan partials, a single mesh's
+  // boundary flags); anything beyond it comes from the pool as before
+  {
+    auto line = [](int64_t n, size_t elem) { return ((size_t)std::max<int64_t>(n, 1) * elem + 255) & ~(size_t)255; };
+    size_t est = line(L.nnz, 4) + line(L.nv, 4) + line(L.nv + 1, 4) + line(L.nnz, 4) + 2 * line(L.nnz, 8) +
+                 line(L.nv, 4) + line(L.nv + 1, 4) + line(L.nnz, 4);
+    if (!L.quad) est += line(L.nf + 1, 4) + line(L.nnz, 4);
+    if (L.segs.size() != 1) est += line(L.nv, 16);
+    if (scheme_ == kSLAK) est += line(L.nnz, 4);
+    est += (size_t)256 << 10;
+    arena_.reserve(est);
+  }
   int32_t* ids_pos = arena_.get_n<int32_t>(L.nnz);
   L.ids = ids_pos;
   if (!L.quad) {
