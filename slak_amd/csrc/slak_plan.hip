@@ -509,36 +509,6 @@ void Plan::prepare() {
     }
   }
 
-  // one arena block for the level buffers that follow (positions and ids
-  // handed on, the positions for k_final, the alternating structure, the
-  // boundary flags and their scan, the tables), estimated from the
-  // closed-form sizes; anything beyond it comes from the pool as before
-  if (Lmax >= 1) {
-    auto line = [](int64_t n, size_t elem) { return ((size_t)std::max<int64_t>(n, 1) * elem + 255) & ~(size_t)255; };
-    size_t est = 0;
-    int64_t mnv = 0, mnnz = 0, mne = 0;
-    for (int l = 1; l <= Lmax; ++l) {
-      int64_t nv = 0, nnz = 0, ne = 0, pad = 0;
-      for (int j : levels_[0].jobs) {
-        const MeshJob& J = jobs_[j];
-        if (J.levels > l) {
-          nv += J.nv[l];
-          nnz += J.nnz[l];
-          pad += 8;
-        }
-        if (J.levels >= l) ne += J.ne[l - 1];
-      }
-      est += line(nv + pad, 16) + 256 + line(nnz, 4);
-      mnv = std::max(mnv, nv);
-      mnnz = std::max(mnnz, nnz);
-      mne = std::max(mne, ne);
-    }
-    est += 2 * (line(mnv + 1, 4) + 2 * line(mnnz, 8) + line(mnnz, 4));
-    est += 2 * line(mne + 1, 4) + line(scan_partials_len(mne), 8);
-    est += (size_t)1 << 20;
-    arena_.reserve(est);
-  }
-
   // 4. level unions, destinations and outputs
   const bool lak_scheme = scheme_ == kLAK;
   auto out_alloc = [&](size_t bytes) -> void* {
@@ -610,6 +580,41 @@ void Plan::prepare() {
       J.p1 = nullptr;
       fused_final_ |= J.fused;
     }
+  }
+  // one arena block for the level buffers that follow (positions and ids
+  // handed on, the positions for k_final, the alternating structure, the
+  // boundary flags and their scan, the tables), from the closed-form sizes
+  // of what the level loop below allocates; anything beyond it comes from
+  // the pool as before
+  {
+    auto line = [](int64_t n, size_t elem) { return ((size_t)std::max<int64_t>(n, 1) * elem + 255) & ~(size_t)255; };
+    size_t est = 0;
+    int64_t mnv = 0, mnnz = 0, mne = 0;
+    for (int l = 1; l <= Lmax; ++l) {
+      int64_t nv = 0, pad = 0, ids = 0, snv = 0, snnz = 0, ne = 0;
+      for (int j : levels_[0].jobs) {
+        const MeshJob& J = jobs_[j];
+        const bool to_final = J.fused && J.levels == l + 1;
+        if (J.levels > l) {  // positions handed on (to the union or to k_final)
+          nv += J.nv[l];
+          pad += 8;
+          if (!to_final) ids += J.nnz[l];
+        }
+        if (J.levels > l && !to_final) {  // index l's structure: level l + 1 runs it
+          snv += J.nv[l];
+          snnz += J.nnz[l];
+        }
+        if (J.levels >= l) ne += J.ne[l - 1];
+      }
+      est += line(nv + pad, 16) + 256 + line(ids, 4);
+      mnv = std::max(mnv, snv);
+      mnnz = std::max(mnnz, snnz);
+      mne = std::max(mne, ne);
+    }
+    est += 2 * (line(mnv + 1, 4) + 2 * line(mnnz, 8) + line(mnnz, 4));
+    est += 2 * line(mne + 1, 4) + line(scan_partials_len(mne), 8);
+    est += (size_t)1 << 20;
+    if (Lmax >= 1) arena_.reserve(est);
   }
   // level l (1-based) consumes sizes index l-1 and produces index l
   int64_t max_nv = 0, max_nnz = 0;
