@@ -430,8 +430,9 @@ def main():
         pmc_child(args.workload, args.scheme)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (meshes are independent; "
-              "see DESIGN.md for how the work would shard). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (meshes are independent and would be "
+              "spread over GPUs by projected output faces; one huge mesh would be cut into face blocks "
+              "with a one-ring halo). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     # HBM counters first, in child processes, before this one touches the GPU
