@@ -85,16 +85,21 @@ def test_batch_equals_single_mesh_calls(oracle, scheme, monkeypatch):
 
 
 @pytest.mark.parametrize("scheme", [LAK, SLAK], ids=["LAK", "SLAK"])
-@pytest.mark.parametrize("depth", [2, 3])
+@pytest.mark.parametrize("depth,quad_only", [(2, True), (3, False), (3, True), (4, True)],
+                         ids=["d2", "d3_poly", "d3_quad", "d4_quad"])
 @pytest.mark.parametrize("use_graph", [False, True], ids=["eager", "graph"])
-def test_same_depth_batch_last_level_in_one_launch(oracle, scheme, depth, use_graph, monkeypatch):
+def test_same_depth_batch_last_level_in_one_launch(oracle, scheme, depth, quad_only, use_graph, monkeypatch):
     """Every mesh refined to the same level: the batch's last level is formed
     for all meshes at once from the quad meshes two levels back (several
-    meshes per corner block, open and closed, polygonal inputs at depth 3).
-    Each mesh must equal its single-mesh result bit for bit."""
+    meshes per corner block, open and closed, polygonal inputs at depth 3),
+    and with quad inputs at depth 3 and 4 the level before it too (positions
+    only, from the inputs' own structure at depth 3). Each mesh must equal its
+    single-mesh result bit for bit."""
     hosts, _ = mixed_batch(30, seed=11)
-    if depth == 2:  # two levels: the inputs themselves must be quads
+    if quad_only:
         hosts = [h for h in hosts if h.type == g.QUAD]
+    if depth == 4:  # keep the outputs small
+        hosts = hosts[:8]
     hosts = hosts + [g.open_grid(3, 2, sigma=0.01, seed=5, mesh=99), g.torus(5, 4, seed=6, mesh=98)]
     levels = [depth] * len(hosts)
     dev = [MeshInfo.from_host(h) for h in hosts]
