@@ -422,6 +422,8 @@ __device__ __forceinline__ void row_number(Row& r, const LevelView& L, int32_t v
 }
 
 // One level's refinement inputs and the next level's structure.
+constexpr int kMaxVSeg = 8;
+
 struct RefineView {
   const int32_t* ids;
   const int32_t* fptr;   // poly levels
@@ -444,9 +446,13 @@ struct RefineView {
   int32_t* nmpos;        // next level's M^T positions (SLAK)
   int32_t next_nv;       // next level's union vertices and corners (end of its M^T)
   int32_t next_nnz;
+  // one mesh: its vertex items as up to kMaxVSeg class runs [vseg[s],
+  // vseg[s+1]) of vnb[s] blocks, each run spread in proportion over the
+  // launch (nvseg 0: one run)
+  int32_t nvseg;
+  int32_t vseg[kMaxVSeg + 1];
+  int32_t vnb[kMaxVSeg];
 };
-
-constexpr int kMaxVSeg = 8;
 
 // One mesh of a final level computed from the structure of the quad mesh M
 // two levels before the output and the positions of its refinement M1 (whose

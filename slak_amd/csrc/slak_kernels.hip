@@ -44,13 +44,15 @@ constexpr int kRowCap = 8;  // M^T rows up to this length stay in registers
 // With the remap, consecutive tiles (which gather from neighbouring parts of
 // the mesh) land on the same XCD: XCD x runs the x-th eighth of the grid in
 // order. Selected once per process by the bit mask SLAK_XCD_REMAP: bit 0 for
-// the level kernels (measured 10-25 % slower on the 256x128 torus chains),
-// bit 1 for k_final (its gathers then stay in one L2: 136 -> 113 us on the
-// 4-level chain of that torus). Default 2.
+// the level kernels that hand on a structure, bit 1 for k_final (its gathers
+// then stay in one L2: 136 -> 113 us on the 4-level chain of the 256x128
+// torus), bit 2 for the other level kernels, bit 3 for every other kernel.
+// For the level kernels it pays together with vertex items in class runs
+// (alone it measured 10-25 % slower). Default 3.
 __device__ int g_xcd_remap;
-constexpr int kRemapLevel = 1, kRemapFinal = 2, kRemapPositions = 4;
+constexpr int kRemapLevel = 1, kRemapFinal = 2, kRemapPositions = 4, kRemapOther = 8;
 
-template <int BIT = kRemapLevel>
+template <int BIT = kRemapOther>
 __device__ __forceinline__ int64_t block_index() {
   const int b = blockIdx.x;
   if (!(g_xcd_remap & BIT)) return b;
@@ -767,10 +769,25 @@ __global__ void __launch_bounds__(kBlock) k_level(RefineView R, int32_t nbr, int
   const int64_t b = NEXT ? block_index<kRemapLevel>() : block_index<kRemapPositions>();
   const int64_t T = (int64_t)nbr + nbv;
   const int64_t before = b * nbv / T, after = (b + 1) * nbv / T;
-  if (after > before)
-    vertex_item<QUAD, SCHEME, NEXT, BND>(R, (int32_t)(before * blockDim.x + threadIdx.x));
-  else
+  if (after > before) {
+    // vertex block `before`: its class run and its block within the run
+    int64_t u = before, rest = nbv;
+    int s = 0;
+    for (; s + 1 < R.nvseg; ++s) {
+      const int64_t nb = R.vnb[s];
+      const int64_t lo = u * nb / rest, hi = (u + 1) * nb / rest;
+      if (hi > lo) {
+        u = lo;
+        break;
+      }
+      u -= lo;
+      rest -= nb;
+    }
+    const int64_t v = (int64_t)R.vseg[s] + u * blockDim.x + threadIdx.x;
+    vertex_item<QUAD, SCHEME, NEXT, BND>(R, v < R.vseg[s + 1] ? (int32_t)v : R.nv);
+  } else {
     refine_item<QUAD, SCHEME>(R, (b - before) * blockDim.x + threadIdx.x);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1431,7 +1448,7 @@ void init_device_options() {
   static std::once_flag once;
   std::call_once(once, [] {
     const char* e = std::getenv("SLAK_XCD_REMAP");
-    int on = e ? std::atoi(e) : kRemapFinal;
+    int on = e ? std::atoi(e) : (kRemapLevel | kRemapFinal);
     (void)hipMemcpyToSymbol(HIP_SYMBOL(g_xcd_remap), &on, sizeof(int));
   });
 }
@@ -1568,9 +1585,26 @@ void launch_level_t(const RefineView& R, unsigned g, int32_t nbr, int32_t nbv, h
   }
 }
 
-void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st) {
+void launch_level(bool quad, int scheme, const RefineView& R0, hipStream_t st) {
+  RefineView R = R0;
+  static const bool runs = [] {  // SLAK_LEVEL_RUNS=0: vertex items in plain order
+    const char* e = std::getenv("SLAK_LEVEL_RUNS");
+    return !e || std::atoi(e) != 0;
+  }();
+  if (!runs || R.nvseg <= 0 || R.nvseg > kMaxVSeg || R.vseg[0] != 0 || R.vseg[R.nvseg] != R.nv) {
+    R.nvseg = 1;
+    R.vseg[0] = 0;
+    R.vseg[1] = R.nv;
+  }
   const int32_t nbr = R.nnz > 0 ? (int32_t)grid_for(R.nnz) : 0;
-  const int32_t nbv = R.nv > 0 ? (int32_t)grid_for(R.nv) : 0;
+  int32_t nbv = 0;
+  for (int s = 0; s < R.nvseg; ++s) {
+    const int64_t n = (int64_t)R.vseg[s + 1] - R.vseg[s];
+    if (n < 0) throw Error(5, "level: vertex runs out of order");  // SLAK_ERR_INTERNAL
+    R.vnb[s] = (int32_t)((n + kBlock - 1) / kBlock);
+    nbv += R.vnb[s];
+  }
+  if (R.nv <= 0) nbv = 0;
   if (nbr + nbv == 0) return;
   const unsigned g = (unsigned)(nbr + nbv);
   if (quad) {

@@ -1049,6 +1049,18 @@ void Plan::record_level(int l) {
   R.nmpos = L.nmpos;
   R.next_nv = (int32_t)L.next_nv;
   R.next_nnz = (int32_t)L.next_nnz;
+  if (L.segs.size() == 1 && l >= 2) {
+    // one mesh: its vertex classes (the original vertices, then the face and
+    // edge points of every earlier level) as runs in step with the corners
+    std::vector<int64_t> cls = vertex_classes(jobs_[L.jobs[0]], l + 1);
+    while (cls.size() > (size_t)kMaxVSeg) {
+      cls[1] += cls[0];
+      cls.erase(cls.begin());
+    }
+    R.nvseg = (int32_t)cls.size();
+    R.vseg[0] = 0;
+    for (size_t i = 0; i < cls.size(); ++i) R.vseg[i + 1] = R.vseg[i] + (int32_t)cls[i];
+  }
   const bool next = L.ntptr != nullptr;
   {
     // algorithmic bytes of the fused launch, each array once: reads ids,
