@@ -343,6 +343,106 @@ def cylinder(a: int, b: int, R: float = 1.0, height: float = 1.0) -> HostMesh:
     return HostMesh(QUAD, ptr, q.reshape(-1).copy(), _vals_from_ptr(ptr), verts, label=f"cyl{a}x{b}")
 
 
+def _mesh_from_faces(faces, pos: np.ndarray, sigma: float, seed: int, mesh: int, label: str) -> HostMesh:
+    """HostMesh from a list of faces (vertex id lists, consistently oriented)
+    and float64 positions [nv, 3], with N(0, sigma^2) jitter from stream
+    (seed, mesh, 0)."""
+    sizes = np.array([len(f) for f in faces], dtype=np.int64)
+    ptr = np.zeros(len(faces) + 1, dtype=np.int64)
+    np.cumsum(sizes, out=ptr[1:])
+    ids = np.concatenate([np.asarray(f, dtype=np.int64) for f in faces]).astype(np.int32)
+    nv = pos.shape[0]
+    p = pos.astype(np.float64).copy()
+    if sigma:
+        p += sigma * gaussians(seed, mesh, 0, 3 * nv).reshape(nv, 3)
+    verts = np.ones((nv, 4), np.float32)
+    verts[:, :3] = p.astype(np.float32)
+    ptr = ptr.astype(np.int32)
+    return HostMesh(classify(ptr), ptr, ids, _vals_from_ptr(ptr), verts, label=label)
+
+
+def uv_sphere(segments: int, rings: int, sigma: float = 1e-3, seed: int = 1, mesh: int = 0) -> HostMesh:
+    """Closed UV sphere: ``rings - 1`` rings of ``segments`` vertices between
+    two poles, quads between rings and a triangle fan at each pole, so each
+    pole has valence ``segments`` (the high-valence rows of the mapped SpGEMM).
+    Vertex 0 is the north pole, ring k (k = 1 .. rings-1) holds vertices
+    1 + (k-1) segments + i, the south pole is last."""
+    if segments < 3 or rings < 2:
+        raise ValueError("uv_sphere needs segments >= 3, rings >= 2")
+    n = segments
+    nring = rings - 1
+    nv = 2 + nring * n
+    south = nv - 1
+
+    def r(k, i):
+        return 1 + (k - 1) * n + (i % n)
+
+    faces = [[0, r(1, i), r(1, i + 1)] for i in range(n)]
+    for k in range(1, nring):
+        faces += [[r(k, i), r(k + 1, i), r(k + 1, i + 1), r(k, i + 1)] for i in range(n)]
+    faces += [[south, r(nring, i + 1), r(nring, i)] for i in range(n)]
+    pos = np.zeros((nv, 3))
+    pos[0] = (0.0, 0.0, 1.0)
+    pos[south] = (0.0, 0.0, -1.0)
+    for k in range(1, nring + 1):
+        th = math.pi * k / rings
+        for i in range(n):
+            ph = 2.0 * math.pi * i / n
+            pos[r(k, i)] = (math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th))
+    return _mesh_from_faces(faces, pos, sigma, seed, mesh, f"uvsphere{segments}x{rings}")
+
+
+def capped_cylinder(segments: int, rows: int, sigma: float = 1e-3, seed: int = 1, mesh: int = 0) -> HostMesh:
+    """Closed prism-like cylinder: ``rows`` rings of quads around, closed by
+    two faces of order ``segments`` (the long faces); every vertex has
+    valence 3."""
+    n = segments
+    nv = n * (rows + 1)
+    faces = [[j * n + i, j * n + (i + 1) % n, (j + 1) * n + (i + 1) % n, (j + 1) * n + i]
+             for j in range(rows) for i in range(n)]
+    faces.append([n - 1 - i for i in range(n)])                  # bottom cap
+    faces.append([rows * n + i for i in range(n)])               # top cap
+    pos = np.zeros((nv, 3))
+    for j in range(rows + 1):
+        for i in range(n):
+            ph = 2.0 * math.pi * i / n
+            pos[j * n + i] = (math.cos(ph), math.sin(ph), j / max(rows, 1) * 2.0 - 1.0)
+    return _mesh_from_faces(faces, pos, sigma, seed, mesh, f"cappedcyl{segments}x{rows}")
+
+
+def fan_disk(segments: int, sigma: float = 1e-3, seed: int = 1, mesh: int = 0) -> HostMesh:
+    """Open disk: one ring of ``segments`` triangles around a centre vertex of
+    valence ``segments``; the rim is a boundary loop."""
+    n = segments
+    faces = [[0, 1 + i, 1 + (i + 1) % n] for i in range(n)]
+    pos = np.zeros((n + 1, 3))
+    for i in range(n):
+        ph = 2.0 * math.pi * i / n
+        pos[1 + i] = (math.cos(ph), math.sin(ph), 0.0)
+    return _mesh_from_faces(faces, pos, sigma, seed, mesh, f"fandisk{segments}")
+
+
+def high_valence_mix(seed: int = 1) -> HostMesh:
+    """One closed mesh with every row length the first level meets from 3 to
+    24 and faces of order 3 to 12: a UV sphere with 24-triangle fans at the
+    poles, a 12-gon-capped cylinder and a 9-segment sphere as separate
+    components of one mesh."""
+    parts = [uv_sphere(24, 6, seed=seed, mesh=0), capped_cylinder(12, 3, seed=seed, mesh=1),
+             uv_sphere(9, 4, seed=seed, mesh=2)]
+    ptr, ids, verts, off, cbase = [np.zeros(1, np.int64)], [], [], 0, 0
+    for k, m in enumerate(parts):
+        ptr.append(m.ptr[1:].astype(np.int64) + cbase)
+        ids.append(m.ids.astype(np.int64) + off)
+        v = m.verts.copy()
+        v[:, 0] += 3.0 * k
+        verts.append(v)
+        off += m.nverts
+        cbase += m.nnz
+    ptr = np.concatenate(ptr).astype(np.int32)
+    return HostMesh(classify(ptr), ptr, np.concatenate(ids).astype(np.int32), _vals_from_ptr(ptr),
+                    np.concatenate(verts), label="high_valence_mix")
+
+
 # ---------------------------------------------------------------------------
 # workload configurations
 

@@ -65,8 +65,17 @@ SMALL = [
     lambda: g.torus_mixed(5, 4, 0.5, pentagons=False, seed=3),
 ]
 
+# rows longer than the register capacities (valence 9..24) and faces of order 12
+HIGH_VALENCE = [
+    lambda: g.uv_sphere(24, 4),
+    lambda: g.uv_sphere(9, 3),
+    lambda: g.capped_cylinder(12, 2),
+    lambda: g.fan_disk(16),
+    lambda: g.high_valence_mix(),
+]
 
-@pytest.mark.parametrize("make", SMALL)
+
+@pytest.mark.parametrize("make", SMALL + HIGH_VALENCE)
 @pytest.mark.parametrize("scheme", [LAK, SLAK])
 def test_one_level_matches_textbook(oracle, make, scheme):
     m = make()
@@ -84,7 +93,7 @@ def test_one_level_matches_textbook(oracle, make, scheme):
         assert r.ptr is None and r.vals is None
 
 
-@pytest.mark.parametrize("make", SMALL[:6])
+@pytest.mark.parametrize("make", SMALL[:6] + HIGH_VALENCE)
 def test_spgemm_matches_dense_brute_force(oracle, make):
     m = make()
     faces = R.faces_of(m.ptr, m.ids)
@@ -145,7 +154,7 @@ def test_chain_equals_iterated_levels(oracle):
     np.testing.assert_array_equal(full.P.astype(np.float32), cur.verts)
 
 
-@pytest.mark.parametrize("make", SMALL)
+@pytest.mark.parametrize("make", SMALL + HIGH_VALENCE)
 def test_lak_and_slak_agree_up_to_edge_permutation(oracle, make):
     m = make()
     lak = run(oracle, m, 2, LAK)
@@ -161,7 +170,7 @@ def test_lak_and_slak_agree_up_to_edge_permutation(oracle, make):
     assert np.abs(slak.P - lak.P[p]).max() < 1e-13
 
 
-@pytest.mark.parametrize("make", SMALL)
+@pytest.mark.parametrize("make", SMALL + HIGH_VALENCE)
 def test_refined_topology_invariants(oracle, make):
     m = make()
     r = run(oracle, m, 2, LAK)
