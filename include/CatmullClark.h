@@ -7,8 +7,11 @@
 #include "LAKCatmullClark.h"
 
 struct CatmullClark {
+  // A type of its own with LAKCatmullClark::MeshInfo's layout, as in the
+  // reference (LibInclude/CatmullClark.h:69-130).
   template <typename OFFSET_TYPE, typename INDEX_TYPE, typename VALUE_TYPE, typename VERTEX_TYPE>
-  using MeshInfo = LAKCatmullClark::MeshInfo<OFFSET_TYPE, INDEX_TYPE, VALUE_TYPE, VERTEX_TYPE>;
+  struct MeshInfo : slak_dropin::MeshInfoFields<MeshInfo<OFFSET_TYPE, INDEX_TYPE, VALUE_TYPE, VERTEX_TYPE>,
+                                                OFFSET_TYPE, INDEX_TYPE, VALUE_TYPE, VERTEX_TYPE> {};
 
   template <typename MESH_INFO>
   static void subdivideIteratively(MESH_INFO const& cmesh, MESH_INFO& rmesh, int target_level) {

@@ -83,53 +83,61 @@ void subdivide(int scheme, const char* banner, MI const& cmesh, MI& rmesh, int t
   }
 }
 
+// The fields and methods of MeshInfo (LibInclude/LAKCatmullClark.h:95-156,
+// LibInclude/CatmullClark.h:69-130). LAKCatmullClark::MeshInfo and
+// CatmullClark::MeshInfo derive from it with themselves as Self, so they are
+// distinct types with the same layout, as in the reference.
+template <class Self, typename OFFSET_TYPE, typename INDEX_TYPE, typename VALUE_TYPE, typename VERTEX_TYPE>
+struct MeshInfoFields {
+  enum class MeshType { TRI, QUAD, POLY };
+  MeshType type;
+
+  size_t nfaces{0};
+  size_t nverts{0};
+  size_t nnz{0};
+  size_t max_face_order{0};
+
+  bool is_reduced{false};
+
+  using offset_t = OFFSET_TYPE;
+  OFFSET_TYPE* ptr{nullptr};
+
+  using index_t = INDEX_TYPE;
+  INDEX_TYPE* ids{nullptr};
+
+  using value_t = VALUE_TYPE;
+  VALUE_TYPE* vals{nullptr};
+
+  using vertex_t = VERTEX_TYPE;
+  VERTEX_TYPE* verts{nullptr};
+
+  void reset() { static_cast<Self&>(*this) = Self(); }
+
+  void freeAndReset() {
+    if (ptr != nullptr) succeed(slak_free(ptr));
+    if (ids != nullptr) succeed(slak_free(ids));
+    if (vals != nullptr) succeed(slak_free(vals));
+    if (verts != nullptr) succeed(slak_free(verts));
+    static_cast<Self&>(*this) = Self();
+  }
+
+  // Buffers the caller's memory manager tracks go back to it, the rest to the library.
+  template <typename MEM_MANAGER>
+  void freeAndReset(MEM_MANAGER& mem) {
+    if (ptr != nullptr && !mem.freeMemory(ptr)) succeed(slak_free(ptr));
+    if (ids != nullptr && !mem.freeMemory(ids)) succeed(slak_free(ids));
+    if (vals != nullptr && !mem.freeMemory(vals)) succeed(slak_free(vals));
+    if (verts != nullptr && !mem.freeMemory(verts)) succeed(slak_free(verts));
+    static_cast<Self&>(*this) = Self();
+  }
+};
+
 }  // namespace slak_dropin
 
 struct LAKCatmullClark {
   template <typename OFFSET_TYPE, typename INDEX_TYPE, typename VALUE_TYPE, typename VERTEX_TYPE>
-  struct MeshInfo {
-    enum class MeshType { TRI, QUAD, POLY };
-    MeshType type;
-
-    size_t nfaces{0};
-    size_t nverts{0};
-    size_t nnz{0};
-    size_t max_face_order{0};
-
-    bool is_reduced{false};
-
-    using offset_t = OFFSET_TYPE;
-    OFFSET_TYPE* ptr{nullptr};
-
-    using index_t = INDEX_TYPE;
-    INDEX_TYPE* ids{nullptr};
-
-    using value_t = VALUE_TYPE;
-    VALUE_TYPE* vals{nullptr};
-
-    using vertex_t = VERTEX_TYPE;
-    VERTEX_TYPE* verts{nullptr};
-
-    void reset() { *this = MeshInfo(); }
-
-    void freeAndReset() {
-      if (ptr != nullptr) slak_dropin::succeed(slak_free(ptr));
-      if (ids != nullptr) slak_dropin::succeed(slak_free(ids));
-      if (vals != nullptr) slak_dropin::succeed(slak_free(vals));
-      if (verts != nullptr) slak_dropin::succeed(slak_free(verts));
-      *this = MeshInfo();
-    }
-
-    // Buffers the caller's memory manager tracks go back to it, the rest to the library.
-    template <typename MEM_MANAGER>
-    void freeAndReset(MEM_MANAGER& mem) {
-      if (ptr != nullptr && !mem.freeMemory(ptr)) slak_dropin::succeed(slak_free(ptr));
-      if (ids != nullptr && !mem.freeMemory(ids)) slak_dropin::succeed(slak_free(ids));
-      if (vals != nullptr && !mem.freeMemory(vals)) slak_dropin::succeed(slak_free(vals));
-      if (verts != nullptr && !mem.freeMemory(verts)) slak_dropin::succeed(slak_free(verts));
-      *this = MeshInfo();
-    }
-  };
+  struct MeshInfo : slak_dropin::MeshInfoFields<MeshInfo<OFFSET_TYPE, INDEX_TYPE, VALUE_TYPE, VERTEX_TYPE>,
+                                                OFFSET_TYPE, INDEX_TYPE, VALUE_TYPE, VERTEX_TYPE> {};
 
   template <typename MESH_INFO>
   static void subdivideIteratively(MESH_INFO const& cmesh, MESH_INFO& rmesh, int target_level) {

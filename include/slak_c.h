@@ -74,8 +74,11 @@ int slak_subdivide(int32_t scheme, const slak_mesh* cmesh, slak_mesh* rmesh, int
                    slak_stats* stats, void* stream);
 
 /* Frees a buffer returned by the library (replaces cudaFree in
- * MeshInfo::freeAndReset, LibInclude/LAKCatmullClark.h:120-127). Like
- * hipFree it waits for the device first. Also accepts slak_malloc pointers. */
+ * MeshInfo::freeAndReset, LibInclude/LAKCatmullClark.h:120-127). It waits for
+ * the legacy default stream (and so for the blocking streams), not for the
+ * whole device: work of the caller's own non-blocking streams that still
+ * reads the buffer must be synchronized first. Also accepts slak_malloc
+ * pointers. */
 int slak_free(void* device_ptr);
 
 /* Message of this thread's last failure ("" if none). */
@@ -106,6 +109,13 @@ int slak_profile_report(char* buf, uint64_t len);
  * *gbps = (read + write bytes) / time. Also the calibration kernel for the
  * FETCH_SIZE / WRITE_SIZE counters. */
 int slak_copy_probe(uint64_t bytes, int32_t iters, double* gbps);
+
+/* Order-sensitive 64-bit digest of `bytes` bytes of device memory: the sum
+ * mod 2^64 over the 32-bit words w_i (the last zero-padded) of
+ * splitmix64(w_i + (i + 1) * 0x9E3779B97F4A7C15). Compares device results
+ * (a batched mesh with the same mesh refined alone) without copying them to
+ * the host. No counterpart in the reference. Synchronous. */
+int slak_digest(const void* device_ptr, uint64_t bytes, uint64_t* digest, void* stream);
 
 /* ---- sparse operators of the LAK method (LibInclude/math/bhblas.h:30-51) ---- */
 
@@ -176,26 +186,30 @@ int slak_batch_destroy(slak_batch* batch);
 /* ---- serving (no counterpart in the reference) ----
  * A request loop with continuous batching: submitted requests are admitted
  * (FIFO) while the sum of their projected device footprints stays within
- * memory_budget_bytes and at most max_batch are in flight; every step
- * advances each admitted request by one level, all of them in the same
- * batched kernels, so requests that arrive while others are on the GPU join
- * at the next step. Intermediate levels stay resident on the device. The
- * submitted mesh must stay valid until the request completes. Results are
- * identical to slak_subdivide of the same mesh. */
+ * memory_budget_bytes, at most max_batch per step. A step refines each of its
+ * requests to the target level in the batched kernels (one launch per kernel
+ * per level over the union of the requests still active at that level; the
+ * first level's analysis once, later levels' structure in closed form, the
+ * last level from the quad mesh two levels back). Up to `slots` steps run at
+ * once, each on its own stream, so requests that arrive while others are on
+ * the GPU start beside them as soon as a slot and memory are free.
+ * Intermediate levels stay resident on the device. The submitted mesh must
+ * stay valid until the request completes. Results are identical to
+ * slak_subdivide of the same mesh. */
 typedef struct slak_server slak_server;
 
 typedef struct slak_server_config {
   int32_t scheme;
   int32_t device;
-  int32_t max_batch;              /* <= 0: 1024 */
-  int32_t reserved;
+  int32_t max_batch;              /* requests per step; <= 0: 1024 */
+  int32_t slots;                  /* concurrent steps, 0..8; 0: 2 */
   uint64_t memory_budget_bytes;   /* 0: 64 GiB */
 } slak_server_config;
 
 typedef struct slak_request_stats {
   int32_t status;
   int32_t levels;
-  int32_t steps;       /* batched steps the request took part in */
+  int32_t steps;       /* batched steps the request took part in (1) */
   int32_t reserved;
   double submit_us;    /* times relative to the server start */
   double admit_us;

@@ -190,6 +190,54 @@ double oracle_time_ms(int scheme, int nthreads, int type, int64_t nf, int64_t nv
   }
 }
 
+// CPU baseline of a batch: the FP32 LAK algorithm on every mesh of a list,
+// each at its own level, the meshes spread over `nthreads` OpenMP threads
+// (dynamic schedule, one single-threaded chain per mesh). Median of `reps`
+// timed passes over the whole list after `warmup` untimed ones; the total
+// final face count goes to *out_faces.
+double oracle_time_batch_ms(int scheme, int nthreads, int nmeshes, const int* types, const int64_t* nf,
+                            const int64_t* nv, const int64_t* nnz, const int32_t* const* ptr,
+                            const int32_t* const* ids, const int32_t* const* vals, const float* const* verts4,
+                            const int* levels, int warmup, int reps, int64_t* out_faces) {
+  try {
+    std::vector<Mesh<float>> in(nmeshes);
+    for (int i = 0; i < nmeshes; ++i)
+      in[i] = make_mesh<float>(types[i], nf[i], nv[i], nnz[i], ptr[i], ids[i], vals ? vals[i] : nullptr, verts4[i]);
+    Options opt;
+    opt.nthreads = 1;
+    std::vector<double> t;
+    std::vector<int64_t> faces(nmeshes, 0);
+    for (int r = 0; r < warmup + reps; ++r) {
+      std::string err;
+      auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+      for (int i = 0; i < nmeshes; ++i) {
+        try {
+          Mesh<float> out;
+          subdivide<float>(scheme, in[i], out, levels[i], opt, nullptr);
+          faces[i] = out.nf;
+        } catch (const std::exception& e) {
+#pragma omp critical
+          err = e.what();
+        }
+      }
+      auto t1 = std::chrono::steady_clock::now();
+      if (!err.empty()) throw std::runtime_error(err);
+      if (r >= warmup) t.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
+    }
+    if (out_faces) {
+      int64_t s = 0;
+      for (int64_t f : faces) s += f;
+      *out_faces = s;
+    }
+    std::sort(t.begin(), t.end());
+    return t.empty() ? 0.0 : t[t.size() / 2];
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1.0;
+  }
+}
+
 int oracle_max_threads() {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -52,6 +52,13 @@ def lib():
                                      ctypes.c_int64, i32p, i32p, i32p, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
         L.oracle_max_threads.restype = ctypes.c_int
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        L.oracle_time_batch_ms.restype = ctypes.c_double
+        L.oracle_time_batch_ms.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                           ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_int64), pp, pp, pp, pp,
+                                           ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_int64)]
         _lib = L
     return _lib
 
@@ -133,6 +140,34 @@ def time_cpu(mesh, levels: int, nthreads: int, scheme: int = LAK, warmup: int = 
     ms = L.oracle_time_ms(scheme, nthreads, mesh.type, mesh.nfaces, mesh.nverts, mesh.nnz,
                           _p(ptr, ctypes.c_int32), _p(ids, ctypes.c_int32), _p(vals, ctypes.c_int32),
                           _p(verts, ctypes.c_float), levels, warmup, reps, ctypes.byref(faces))
+    if ms < 0:
+        raise ValueError(L.oracle_last_error().decode())
+    return ms, int(faces.value)
+
+
+def time_cpu_batch(meshes, levels, nthreads: int, scheme: int = LAK, warmup: int = 1, reps: int = 3):
+    """Median wall ms of the FP32 LAK algorithm over a list of meshes, each at
+    its own level, spread over ``nthreads`` host threads (one mesh per thread
+    at a time). Returns (ms, total final faces)."""
+    L = lib()
+    n = len(meshes)
+    keep = [_inputs(m) for m in meshes]
+    VP = ctypes.c_void_p * n
+
+    def ptrs(k):
+        return VP(*[None if a[k] is None else a[k].ctypes.data for a in keep])
+
+    has_vals = any(a[2] is not None for a in keep)
+    types = (ctypes.c_int * n)(*[m.type for m in meshes])
+    nf = (ctypes.c_int64 * n)(*[m.nfaces for m in meshes])
+    nv = (ctypes.c_int64 * n)(*[m.nverts for m in meshes])
+    nnz = (ctypes.c_int64 * n)(*[m.nnz for m in meshes])
+    lv = (ctypes.c_int * n)(*[int(x) for x in levels])
+    faces = ctypes.c_int64(0)
+    cast = lambda x: ctypes.cast(x, ctypes.POINTER(ctypes.c_void_p))
+    ms = L.oracle_time_batch_ms(scheme, nthreads, n, types, nf, nv, nnz, cast(ptrs(0)), cast(ptrs(1)),
+                                cast(ptrs(2)) if has_vals else None, cast(ptrs(3)), lv, warmup, reps,
+                                ctypes.byref(faces))
     if ms < 0:
         raise ValueError(L.oracle_last_error().decode())
     return ms, int(faces.value)

@@ -65,7 +65,7 @@ class SlakServerConfig(ctypes.Structure):
         ("scheme", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("max_batch", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("slots", ctypes.c_int32),
         ("memory_budget_bytes", ctypes.c_uint64),
     ]
 
@@ -129,6 +129,8 @@ EXPORTS = [
     ("slak_batch_create", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(SlakMesh),
                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_void_p),
                                          ctypes.c_void_p]),
+    ("slak_digest", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+                                   ctypes.c_void_p]),
     ("slak_batch_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     ("slak_batch_result", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(SlakMesh)]),
     ("slak_batch_info_get", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SlakBatchInfo)]),
@@ -169,6 +171,13 @@ def profile_report() -> dict:
     buf = ctypes.create_string_buffer(1 << 20)
     check(lib().slak_profile_report(buf, len(buf)))
     return json.loads(buf.value.decode())
+
+
+def digest(ptr: int, nbytes: int) -> int:
+    """slak_digest: 64-bit order-sensitive digest of device memory."""
+    out = ctypes.c_uint64()
+    check(lib().slak_digest(ptr or None, int(nbytes), ctypes.byref(out), None))
+    return int(out.value)
 
 
 def check(code: int) -> None:

@@ -47,6 +47,8 @@ constexpr int kRowHistBins = 17;  // row lengths 0..15, 16 = longer
 // bounds[k] = eptr[vbase[k]], k = 0..nseg
 void launch_seg_bounds(const int64_t* vbase, int32_t nseg, const int32_t* eptr, int32_t* bounds, hipStream_t st);
 void launch_copy_probe(const float4* a, float4* b, int64_t n, hipStream_t st);
+// *out = order-sensitive 64-bit digest of bytes at p (out: device, 8 bytes)
+void launch_digest(const void* p, int64_t bytes, uint64_t* out, hipStream_t st);
 // guard (or null): the kernel does nothing when it holds an input-validation flag
 void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int32_t* fptr, const float4* P, int32_t nf,
                   const SegTable& seg, int32_t* cnt, const int* guard, hipStream_t st);
@@ -154,6 +156,8 @@ class Profiler {
   void set_enabled(bool on) { enabled_ = on; }
   void begin(hipStream_t st, const char* name, double bytes);
   void end(hipStream_t st);
+  // adds `bytes` to the latest record of stage `name` (names compare by text)
+  void amend_last(const char* name, double bytes);
   void reset();
   std::string report();  // JSON
 

@@ -87,6 +87,44 @@ __global__ void __launch_bounds__(kBlock) k_copy_probe(const float4* __restrict_
   if (i < n) b[i] = a[i];
 }
 
+// Order-sensitive 64-bit digest of a byte buffer: the sum (mod 2^64) over its
+// 32-bit words w_i (the last one zero-padded) of mix64(w_i + (i + 1) * G), a
+// splitmix64 finaliser. A sum is independent of the reduction order, so the
+// digest is deterministic; each block adds its partial to *out (zeroed first).
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+constexpr int kDigestBlocks = 2048;
+
+__global__ void __launch_bounds__(kBlock) k_digest(const uint8_t* __restrict__ p, int64_t bytes,
+                                                   unsigned long long* __restrict__ out) {
+  const int64_t nw = (bytes + 3) / 4, full = bytes / 4;
+  const uint32_t* __restrict__ w = reinterpret_cast<const uint32_t*>(p);
+  uint64_t acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t x;
+    if (i < full) {
+      x = w[i];
+    } else {  // the zero-padded last word
+      x = 0;
+      for (int64_t b = 4 * i; b < bytes; ++b) x |= (uint32_t)p[b] << (8 * (b - 4 * i));
+    }
+    acc += mix64((uint64_t)x + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull);
+  }
+  __shared__ uint64_t part[kBlock / 64];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t s = 0;
+    for (int k = 0; k < kBlock / 64; ++k) s += part[k];
+    atomicAdd(out, (unsigned long long)s);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // first level: validate each caller mesh and gather it into the union in
 // position order (sortKeyValuePairsSegmentedInPlace, LAKCatmullClarkKernels.cuh:147-180)
@@ -1484,6 +1522,14 @@ void launch_copy_probe(const float4* a, float4* b, int64_t n, hipStream_t st) {
   if (n <= 0) return;
   if (n > 0x7fffffffll) throw Error(1, "copy probe too large");  // SLAK_ERR_INVALID_ARGUMENT
   k_copy_probe<<<grid_for(n), kBlock, 0, st>>>(a, b, n);
+}
+
+void launch_digest(const void* p, int64_t bytes, uint64_t* out, hipStream_t st) {
+  k_fill_i32<<<1, 64, 0, st>>>(reinterpret_cast<int32_t*>(out), 2, 0);
+  if (bytes <= 0) return;
+  const int64_t nw = (bytes + 3) / 4;
+  const unsigned g = (unsigned)std::min<int64_t>(kDigestBlocks, (nw + kBlock - 1) / kBlock);
+  k_digest<<<g, kBlock, 0, st>>>(static_cast<const uint8_t*>(p), bytes, reinterpret_cast<unsigned long long*>(out));
 }
 
 void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int32_t* fptr, const float4* P, int32_t nf,

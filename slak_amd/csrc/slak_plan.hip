@@ -111,6 +111,15 @@ Plan::Plan(int scheme, std::vector<MeshJob> jobs, OutputMode mode, hipStream_t s
 }
 
 Plan::~Plan() {
+  // a launch that threw after the LAK forms were queued on the auxiliary
+  // stream: they read the level tables in the arena released below
+  if (lak_forms_pending_) {
+    try {
+      (void)hipStreamSynchronize(aux_stream().stream);
+    } catch (...) {
+      (void)hipDeviceSynchronize();
+    }
+  }
   if (exec_) (void)hipGraphExecDestroy(exec_);
   if (graph_) (void)hipGraphDestroy(graph_);
   if (owned_st_) {
@@ -456,6 +465,11 @@ void Plan::prepare() {
     std::function<void()> overlap;
     if (early) overlap = [this, &L1] { launch_number(L1, early_p1_ != nullptr); };
     fail_if(readback(nullptr, 2 * kRowHistBins + nseg + 1, hb.data(), overlap));
+    // the early numbering was recorded before its edge count was known: its
+    // boundary-flag writes (4 bytes per edge) join its algorithmic bytes now
+    if (early && Profiler::get().enabled())
+      Profiler::get().amend_last(L1.quad ? "number_facepoints_quad" : "number_facepoints_poly",
+                                 4.0 * (double)hb[2 * kRowHistBins + 1]);
     number_early_ = early;
     early_faces_ = early && early_p1_ != nullptr;
     have_hist = true;

@@ -204,6 +204,15 @@ void Profiler::end(hipStream_t st) {
   if (!recs_.empty()) SLAK_HIP_CHECK(hipEventRecord(recs_.back().b, st));
 }
 
+void Profiler::amend_last(const char* name, double bytes) {
+  std::lock_guard<std::mutex> g(mu_);
+  for (auto it = recs_.rbegin(); it != recs_.rend(); ++it)
+    if (std::strcmp(it->name, name) == 0) {
+      it->bytes += bytes;
+      return;
+    }
+}
+
 void Profiler::reset() {
   std::lock_guard<std::mutex> g(mu_);
   for (auto& r : recs_) {
@@ -411,7 +420,11 @@ int slak_subdivide(int32_t scheme, const slak_mesh* cmesh, slak_mesh* rmesh, int
 int slak_free(void* p) {
   return guarded([&] {
     if (!p) return;
-    SLAK_HIP_CHECK(hipDeviceSynchronize());
+    // Every library call returns with its own work complete, so a returned
+    // buffer only needs the caller's work on it to be finished: the legacy
+    // default stream and, through it, the blocking streams. Non-blocking
+    // streams (a server's steps) are not waited for.
+    SLAK_HIP_CHECK(hipStreamSynchronize(nullptr));
     if (!slak::pool().release(p)) SLAK_HIP_CHECK(hipFree(p));
   });
 }
@@ -501,6 +514,29 @@ int slak_copy_probe(uint64_t bytes, int32_t iters, double* gbps) {
   });
 }
 
+int slak_digest(const void* p, uint64_t bytes, uint64_t* digest, void* stream) {
+  return guarded([&] {
+    if (!digest || (!p && bytes)) throw Error(SLAK_ERR_INVALID_ARGUMENT, "null pointer");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    slak::Arena arena(slak::pool());
+    uint64_t* d = arena.get_n<uint64_t>(1);
+    slak::launch_digest(p, (int64_t)bytes, d, st);
+    SLAK_HIP_CHECK(hipMemcpyAsync(digest, d, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    SLAK_HIP_CHECK(hipStreamSynchronize(st));
+  });
+}
+
+// per-face reorder of (ids, vals) and the validity read-back; throws on an
+// invalid mesh (ids and vals may already be reordered in place)
+static void sort_and_check(bool reduce, int32_t* ptr, int32_t stride, slak_mesh* m, int32_t* vals, int* flags,
+                           hipStream_t st) {
+  slak::launch_sort_faces(reduce, ptr, stride, (int32_t)m->nfaces, (int64_t)m->nnz, m->ids, vals, flags, st);
+  int fl = 0;
+  SLAK_HIP_CHECK(hipMemcpyAsync(&fl, flags, sizeof(int), hipMemcpyDeviceToHost, st));
+  SLAK_HIP_CHECK(hipStreamSynchronize(st));
+  if (fl) throw Error(SLAK_ERR_INVALID_MESH, "invalid mesh: ptr does not describe faces of the mesh type's order");
+}
+
 static void convert_homogeneous(slak_mesh* m, bool reduce, hipStream_t st) {
   if (!m) throw Error(SLAK_ERR_INVALID_ARGUMENT, "null mesh");
   if (m->type == SLAK_POLY) throw Error(SLAK_ERR_INVALID_ARGUMENT, "mesh type not homogeneous");
@@ -516,23 +552,35 @@ static void convert_homogeneous(slak_mesh* m, bool reduce, hipStream_t st) {
   slak::launch_fill_i32(flags, 1, 0, st);
   if (reduce) {
     if (!m->vals) throw Error(SLAK_ERR_INVALID_ARGUMENT, "unreduced mesh without vals");
+    sort_and_check(reduce, m->ptr, stride, m, m->vals, flags, st);
   } else {
     // ptr and vals of a reduced mesh may be absent: stride offsets and 1-based
     // corner positions (the reference's unreduce fills vals with (c - 1) mod
     // order, i.e. -1, 0, 1, ..., which no LAK kernel accepts as positions)
+    // new buffers stay the library's until the conversion has succeeded
     int32_t* ptr = nullptr;
     int32_t* vals = nullptr;
+    struct Owned {
+      int32_t*& ptr;
+      int32_t*& vals;
+      hipStream_t st;
+      ~Owned() {
+        if (!ptr && !vals) return;
+        (void)hipStreamSynchronize(st);
+        if (ptr) slak::pool().release(ptr);
+        if (vals) slak::pool().release(vals);
+      }
+    } owned{ptr, vals, st};
     if (!m->ptr) ptr = static_cast<int32_t*>(slak::pool().alloc(((size_t)m->nfaces + 1) * sizeof(int32_t)));
     if (!m->vals) vals = static_cast<int32_t*>(slak::pool().alloc(std::max<size_t>(m->nnz, 1) * sizeof(int32_t)));
     slak::launch_fill_homogeneous(ptr, (int64_t)m->nfaces + 1, vals, (int64_t)m->nnz, stride, st);
+    int32_t* use_ptr = ptr ? ptr : m->ptr;
+    int32_t* use_vals = vals ? vals : m->vals;
+    sort_and_check(reduce, use_ptr, stride, m, use_vals, flags, st);
     if (ptr) m->ptr = ptr;
     if (vals) m->vals = vals;
+    ptr = vals = nullptr;  // handed to the caller's mesh
   }
-  slak::launch_sort_faces(reduce, m->ptr, stride, (int32_t)m->nfaces, (int64_t)m->nnz, m->ids, m->vals, flags, st);
-  int fl = 0;
-  SLAK_HIP_CHECK(hipMemcpyAsync(&fl, flags, sizeof(int), hipMemcpyDeviceToHost, st));
-  SLAK_HIP_CHECK(hipStreamSynchronize(st));
-  if (fl) throw Error(SLAK_ERR_INVALID_MESH, "invalid mesh: ptr does not describe faces of the mesh type's order");
   m->is_reduced = reduce ? 1 : 0;
 }
 

@@ -1,14 +1,15 @@
 // Serving: a request loop that refines subdivision requests with continuous
 // batching. Requests are admitted while their projected device footprint fits
-// the memory budget; every step advances each admitted request by one level,
-// all of them together in the batched level kernels (one launch per kernel
-// over the union), so requests that arrive while others are on the GPU join
-// at the next step. Intermediate levels stay resident on the device; a
-// finished request's refined mesh is handed to the caller. No counterpart in
-// the reference, which refines one mesh per blocking call.
+// the memory budget; a step refines every admitted request to its target
+// level in the batched level kernels (one launch per kernel per level over
+// the union of the requests still active at that level), and several steps
+// run at once, one per slot, so requests that arrive while others are on the
+// GPU start beside them. Intermediate levels and their structure stay
+// resident on the device; a finished request's refined mesh is handed to the
+// caller. No counterpart in the reference, which refines one mesh per
+// blocking call.
 #include <hip/hip_runtime.h>
 
-#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstdlib>
@@ -39,211 +40,229 @@ struct Request {
   slak_mesh in{};
   int target = 0;
   int levels = 0;
-  int done_levels = 0;
   uint64_t footprint = 0;
-  // current resident mesh (after done_levels > 0)
+  Clock::time_point t_submit, t_admit;
+  // result: written by the worker under the server mutex together with
+  // `finished`, read by the waiter under the same mutex
   slak_mesh cur{};
-  int64_t cur_ne = -1;  // edges of cur (closed form)
-  Clock::time_point t_submit, t_admit, t_done;
+  Clock::time_point t_done;
   int status = SLAK_OK;
   std::string error;
   bool finished = false;
   int steps = 0;
 };
 
-// Upper bound of the device bytes a request needs at its largest level:
-// input + output positions and ids plus the level scratch (edge count bounded
-// by the corner count).
-uint64_t footprint(const slak_mesh& m, int levels) {
+// Closed-form sizes of a request's levels, with the edge count bounded by the
+// corner count: n_f' = nnz, nnz' = 4 nnz, n_v' = n_v + n_f + n_e.
+struct ChainBound {
+  double peak_bytes = 0;   // device bytes while the chain runs
+  double last_corners = 0; // corners of the input of the last level
+  double last_verts = 0;   // vertices of the input of the last level
+};
+
+ChainBound chain_bound(const slak_mesh& m, int levels) {
+  ChainBound cb;
   double nv = (double)m.nverts, nf = (double)m.nfaces, nnz = (double)m.nnz;
+  double held = 16 * nv + 4 * nnz;  // the caller's input (gathered once)
   double peak = 0;
   for (int l = 0; l < std::max(levels, 1); ++l) {
-    double ne = nnz;  // bound
-    double nv2 = nv + nf + ne, nnz2 = 4 * nnz;
-    double b = 16 * nv + 4 * nnz                  // input
-               + 16 * nv2 + 4 * nnz2 * 2 + 4 * nnz  // output ids, LAK ptr/vals
-               + 16 * nv + 28 * nnz;                // scratch: M^T, rows, (edge, twin), flags
-    peak = std::max(peak, b);
+    const double ne = nnz;  // bound
+    const double nv2 = nv + nf + ne, nnz2 = 4 * nnz;
+    cb.last_corners = nnz;
+    cb.last_verts = nv;
+    // this level's input structure (M^T rows with in-neighbours, (edge,
+    // twin) per corner, boundary flags) and its refined positions and ids
+    // (with the LAK form at the last level)
+    const double level = 28 * nnz + 8 * nv + 16 * nv2 + 4 * nnz2 * 2 + 4 * nnz;
+    peak = std::max(peak, held + level);
+    held += 16 * nv2 + 4 * nnz2;  // a level stays resident while the next one runs
     nv = nv2;
     nf = nnz;
     nnz = nnz2;
   }
-  return (uint64_t)peak;
+  cb.peak_bytes = peak;
+  return cb;
 }
 
 }  // namespace
 
+// Continuous batching over `slots` concurrent steps. A step takes the pending
+// requests that fit (FIFO, by projected device footprint and max_batch) and
+// refines every one of them to its target level in one multi-mesh plan: one
+// launch per kernel per level over the union of the requests still active at
+// that level, the first level's analysis once per request, every later
+// level's structure in closed form from the level before it, the last level
+// from the quad mesh two levels back. Each slot has its own worker thread
+// and stream, so a step starts as soon as requests are pending and a slot is
+// free: requests that arrive while another step is on the GPU run beside it.
 struct slak_server {
   slak_server_config cfg{};
-  hipStream_t stream = nullptr;
+  int nslots = 2;
+  std::vector<hipStream_t> streams;
   std::mutex mu;
   std::condition_variable cv, done_cv;
   std::deque<std::shared_ptr<Request>> pending;
-  std::vector<std::shared_ptr<Request>> active;
   std::map<uint64_t, std::shared_ptr<Request>> all;
   uint64_t next_id = 1;
   uint64_t inflight_bytes = 0;
+  int running = 0;  // steps on the GPU
   bool stop = false;
-  std::thread worker;
+  std::vector<std::thread> workers;
   Clock::time_point t0 = Clock::now();
-  // counters
-  uint64_t steps = 0, completed = 0, faces_done = 0;
+  // counters (under mu)
+  uint64_t steps = 0, completed = 0, faces_done = 0, active = 0;
   double busy_us = 0;
   int max_batch_seen = 0;
-  // refined corners (and vertices) one level plan of a step may hold: the
-  // int32 index range, or less with SLAK_SERVE_STEP_CORNERS
+  // corners (and vertices) one plan's union may hold at any level: the int32
+  // index range, or less with SLAK_SERVE_STEP_CORNERS
   double step_limit = 2147483647.0;
 
-  void loop();
-  void step(std::vector<std::shared_ptr<Request>>& batch);
-  void step_plan(std::vector<std::shared_ptr<Request>>& batch);
+  void loop(int slot);
+  bool admissible(const Request& r) const {
+    return inflight_bytes == 0 || inflight_bytes + r.footprint <= cfg.memory_budget_bytes;
+  }
+  void step(std::vector<std::shared_ptr<Request>>& batch, hipStream_t st);
+  void run_plan(std::vector<std::shared_ptr<Request>>& batch, hipStream_t st);
+  void publish(Request& r, const slak_mesh& m, int status, const std::string& err, Clock::time_point t);
 };
 
-// A level plan indexes its union of meshes with int32: the step runs the
-// batch as consecutive sub-batches whose refined corners and vertices fit.
-void slak_server::step(std::vector<std::shared_ptr<Request>>& batch) {
+// Sets a request's result and marks it finished, under the mutex the waiters
+// read it with.
+void slak_server::publish(Request& r, const slak_mesh& m, int status, const std::string& err, Clock::time_point t) {
+  std::lock_guard<std::mutex> lk(mu);
+  r.cur = m;
+  r.status = status;
+  r.error = err;
+  r.t_done = t;
+  r.steps += 1;
+  r.finished = true;
+  inflight_bytes -= r.footprint;
+  --active;
+  ++completed;
+  if (status == SLAK_OK) faces_done += m.nfaces;
+}
+
+// A plan indexes the union of its meshes with int32 at every level: the step
+// runs the batch as consecutive sub-batches whose deepest level fits.
+void slak_server::step(std::vector<std::shared_ptr<Request>>& batch, hipStream_t st) {
   const double lim = step_limit;
   size_t i = 0;
   while (i < batch.size()) {
     size_t j = i;
     double corners = 0, verts = 0;
     while (j < batch.size()) {
-      const Request& r = *batch[j];
-      const slak_mesh& m = r.done_levels == 0 ? r.in : r.cur;
-      const double c = 4.0 * (double)m.nnz, v = (double)m.nverts + (double)m.nfaces + (double)m.nnz;
+      const ChainBound b = chain_bound(batch[j]->in, batch[j]->levels);
+      const double c = 4.0 * b.last_corners, v = b.last_verts + 2.0 * b.last_corners;
       if (j > i && (corners + c > lim || verts + v > lim)) break;
       corners += c;
       verts += v;
       ++j;
     }
     std::vector<std::shared_ptr<Request>> sub(batch.begin() + i, batch.begin() + j);
-    step_plan(sub);
+    run_plan(sub, st);
     i = j;
   }
 }
 
-void slak_server::step_plan(std::vector<std::shared_ptr<Request>>& batch) {
+void slak_server::run_plan(std::vector<std::shared_ptr<Request>>& batch, hipStream_t st) {
   using namespace slak;
-  std::vector<MeshJob> jobs;
-  jobs.reserve(batch.size());
-  for (auto& r : batch) {
-    MeshJob J;
-    if (r->done_levels == 0) {
+  while (!batch.empty()) {
+    std::vector<MeshJob> jobs;
+    jobs.reserve(batch.size());
+    for (auto& r : batch) {
+      MeshJob J;
       J.in = r->in;
-    } else {
-      J.in = r->cur;
-      J.trusted = true;
-      J.known_ne = r->cur_ne;
+      J.target = r->target;
+      jobs.push_back(J);
     }
-    J.target = 1;
-    J.lak_form = r->done_levels + 1 == r->levels;
-    jobs.push_back(J);
-  }
-  auto t_step = Clock::now();
-  Plan plan(cfg.scheme, std::move(jobs), OutputMode::kSeparate, stream);
-  try {
-    plan.prepare();
-  } catch (const Error& e) {
-    // fail the requests whose meshes were rejected, keep the rest for the next step
-    bool named = false;
-    for (size_t i = 0; i < batch.size(); ++i)
-      if (plan.jobs()[i].bad_flags) {
-        batch[i]->status = e.code;
-        batch[i]->error = std::string("invalid mesh:") + flags_message(plan.jobs()[i].bad_flags);
-        batch[i]->finished = true;
-        named = true;
+    Plan plan(cfg.scheme, std::move(jobs), OutputMode::kSeparate, st);
+    try {
+      plan.prepare();
+    } catch (const Error& e) {
+      // fail the requests whose meshes were rejected and run the rest again
+      std::vector<std::shared_ptr<Request>> rest;
+      bool named = false;
+      for (size_t i = 0; i < batch.size(); ++i)
+        if (plan.jobs()[i].bad_flags) named = true;
+      for (size_t i = 0; i < batch.size(); ++i) {
+        const int bad = plan.jobs()[i].bad_flags;
+        if (bad || !named)
+          publish(*batch[i], slak_mesh{}, e.code,
+                  bad ? std::string("invalid mesh:") + flags_message(bad) : std::string(e.what()), Clock::now());
+        else
+          rest.push_back(batch[i]);
       }
-    if (!named)
-      for (auto& r : batch) {
-        r->status = e.code;
-        r->error = e.what();
-        r->finished = true;
-      }
-    for (auto& r : batch)
-      if (r->finished) r->t_done = Clock::now();
+      batch.swap(rest);
+      continue;
+    }
+    plan.launch(false);
+    plan.finish();
+    plan.release_outputs();
+    const auto t_end = Clock::now();
+    for (size_t i = 0; i < batch.size(); ++i) {
+      MeshJob& J = plan.jobs()[i];
+      slak_mesh m{};
+      const int L = J.levels;
+      m.type = SLAK_QUAD;
+      m.nfaces = (uint64_t)J.nf[L];
+      m.nnz = (uint64_t)J.nnz[L];
+      m.nverts = (uint64_t)J.nv[L];
+      m.max_face_order = 4;
+      m.ids = J.out_ids;
+      m.verts = reinterpret_cast<float*>(J.out_verts);
+      m.ptr = J.out_ptr;
+      m.vals = J.out_vals;
+      m.is_reduced = cfg.scheme == SLAK_SCHEME_LAK ? 0 : 1;
+      publish(*batch[i], m, SLAK_OK, std::string(), t_end);
+    }
     return;
-  }
-  plan.launch(false);
-  plan.finish();
-  plan.release_outputs();
-  auto t_end = Clock::now();
-  busy_us += us_since(t_step, t_end);
-  for (size_t i = 0; i < batch.size(); ++i) {
-    Request& r = *batch[i];
-    MeshJob& J = plan.jobs()[i];
-    // the previous intermediate level is no longer needed
-    if (r.done_levels > 0) {
-      for (void* p : {(void*)r.cur.ids, (void*)r.cur.verts, (void*)r.cur.ptr, (void*)r.cur.vals})
-        if (p) pool().release(p);
-    }
-    slak_mesh m{};
-    m.type = SLAK_QUAD;
-    m.nfaces = (uint64_t)J.nf[1];
-    m.nnz = (uint64_t)J.nnz[1];
-    m.nverts = (uint64_t)J.nv[1];
-    m.max_face_order = 4;
-    m.ids = J.out_ids;
-    m.verts = reinterpret_cast<float*>(J.out_verts);
-    m.ptr = J.out_ptr;
-    m.vals = J.out_vals;
-    m.is_reduced = cfg.scheme == SLAK_SCHEME_LAK ? 0 : 1;
-    r.cur = m;
-    r.cur_ne = J.ne[1];
-    r.done_levels += 1;
-    r.steps += 1;
-    if (r.done_levels >= r.levels) {
-      r.finished = true;
-      r.t_done = t_end;
-      faces_done += m.nfaces;
-    }
   }
 }
 
-void slak_server::loop() {
+void slak_server::loop(int slot) {
   (void)hipSetDevice(cfg.device);
+  hipStream_t st = streams[slot];
   for (;;) {
     std::vector<std::shared_ptr<Request>> batch;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
-      if (stop && pending.empty() && active.empty()) return;
+      cv.wait(lk, [&] { return (stop && pending.empty()) || (!pending.empty() && admissible(*pending.front())); });
+      if (pending.empty()) return;  // stopping, nothing left
       // admission by projected device footprint, FIFO
-      while (!pending.empty() && (int)active.size() < cfg.max_batch) {
-        auto& r = pending.front();
-        if (!active.empty() && inflight_bytes + r->footprint > cfg.memory_budget_bytes) break;
-        r->t_admit = Clock::now();
-        inflight_bytes += r->footprint;
-        active.push_back(r);
+      const auto now = Clock::now();
+      while (!pending.empty() && (int)batch.size() < cfg.max_batch && admissible(*pending.front())) {
+        auto r = pending.front();
         pending.pop_front();
+        r->t_admit = now;
+        inflight_bytes += r->footprint;
+        batch.push_back(r);
       }
-      batch = active;
+      active += batch.size();
+      ++running;
+      ++steps;
+      max_batch_seen = std::max(max_batch_seen, (int)batch.size());
     }
-    max_batch_seen = std::max(max_batch_seen, (int)batch.size());
+    const auto t_step = Clock::now();
     try {
-      step(batch);
+      step(batch, st);
     } catch (const std::exception& e) {
+      (void)hipStreamSynchronize(st);
       for (auto& r : batch) {
-        r->status = SLAK_ERR_INTERNAL;
-        r->error = e.what();
-        r->finished = true;
-        r->t_done = Clock::now();
+        bool done;
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          done = r->finished;
+        }
+        if (!done) publish(*r, slak_mesh{}, SLAK_ERR_INTERNAL, e.what(), Clock::now());
       }
     }
     {
       std::lock_guard<std::mutex> lk(mu);
-      ++steps;
-      std::vector<std::shared_ptr<Request>> still;
-      for (auto& r : active) {
-        if (r->finished) {
-          inflight_bytes -= r->footprint;
-          ++completed;
-        } else {
-          still.push_back(r);
-        }
-      }
-      active.swap(still);
+      busy_us += us_since(t_step, Clock::now());
+      --running;
     }
     done_cv.notify_all();
+    cv.notify_all();  // memory freed: the other slots may admit again
   }
 }
 
@@ -254,6 +273,7 @@ int slak_server_start(const slak_server_config* cfg, slak_server** out) {
     if (!cfg || !out) throw slak::Error(SLAK_ERR_INVALID_ARGUMENT, "null argument");
     if (cfg->scheme != SLAK_SCHEME_LAK && cfg->scheme != SLAK_SCHEME_SLAK)
       throw slak::Error(SLAK_ERR_INVALID_ARGUMENT, "unknown scheme");
+    if (cfg->slots < 0 || cfg->slots > 8) throw slak::Error(SLAK_ERR_INVALID_ARGUMENT, "slots must be in 0..8");
     auto s = std::make_unique<slak_server>();
     s->cfg = *cfg;
     if (const char* e = std::getenv("SLAK_SERVE_STEP_CORNERS")) {
@@ -262,10 +282,24 @@ int slak_server_start(const slak_server_config* cfg, slak_server** out) {
     }
     if (s->cfg.max_batch <= 0) s->cfg.max_batch = 1024;
     if (s->cfg.memory_budget_bytes == 0) s->cfg.memory_budget_bytes = (uint64_t)64 << 30;
+    s->nslots = s->cfg.slots > 0 ? s->cfg.slots : 2;
     SLAK_HIP_CHECK(hipSetDevice(s->cfg.device));
-    SLAK_HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     slak_server* raw = s.get();
-    s->worker = std::thread([raw] { raw->loop(); });
+    auto cleanup = [raw] {
+      for (hipStream_t st : raw->streams) (void)hipStreamDestroy(st);
+      raw->streams.clear();
+    };
+    try {
+      for (int k = 0; k < s->nslots; ++k) {
+        hipStream_t st = nullptr;
+        SLAK_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        s->streams.push_back(st);
+      }
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    for (int k = 0; k < s->nslots; ++k) s->workers.emplace_back([raw, k] { raw->loop(k); });
     *out = s.release();
   });
 }
@@ -278,7 +312,7 @@ int slak_server_submit(slak_server* s, const slak_mesh* mesh, int32_t target_lev
     r->in = *mesh;
     r->target = target_level;
     r->levels = mesh->type != SLAK_QUAD ? std::max(1, (int)target_level) : (int)target_level;
-    r->footprint = footprint(*mesh, r->levels);
+    r->footprint = (uint64_t)chain_bound(*mesh, r->levels).peak_bytes;
     r->t_submit = Clock::now();
     {
       std::lock_guard<std::mutex> lk(s->mu);
@@ -300,7 +334,7 @@ int slak_server_submit(slak_server* s, const slak_mesh* mesh, int32_t target_lev
 }
 
 static void fill_result(const Request& r, slak_mesh* result, slak_request_stats* st, Clock::time_point t0) {
-  if (result && r.status == SLAK_OK && r.levels > 0) *result = r.cur;
+  if (result && r.status == SLAK_OK && r.levels > 0) *result = r.cur;  // finished: read under the mutex
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->status = r.status;
@@ -344,7 +378,7 @@ int slak_server_get_stats(slak_server* s, slak_server_stats* out) {
     out->steps = s->steps;
     out->completed = s->completed;
     out->pending = s->pending.size();
-    out->active = s->active.size();
+    out->active = s->active;
     out->inflight_bytes = s->inflight_bytes;
     out->faces_done = s->faces_done;
     out->busy_us = s->busy_us;
@@ -360,16 +394,17 @@ int slak_server_stop(slak_server* s) {
       s->stop = true;
     }
     s->cv.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    for (auto& w : s->workers)
+      if (w.joinable()) w.join();
     // results nobody collected go back to the library
-    (void)hipStreamSynchronize(s->stream);
+    for (hipStream_t st : s->streams) (void)hipStreamSynchronize(st);
     for (auto& kv : s->all) {
       Request& r = *kv.second;
-      if (r.levels > 0 && r.done_levels > 0)
+      if (r.finished && r.status == SLAK_OK && r.levels > 0)
         for (void* p : {(void*)r.cur.ids, (void*)r.cur.verts, (void*)r.cur.ptr, (void*)r.cur.vals})
           if (p) slak::pool().release(p);
     }
-    (void)hipStreamDestroy(s->stream);
+    for (hipStream_t st : s->streams) (void)hipStreamDestroy(st);
     delete s;
   });
 }

@@ -12,8 +12,8 @@ from .mesh import MeshInfo
 
 class Server:
     def __init__(self, scheme: int = capi.SLAK_SCHEME_LAK, memory_budget_bytes: int = 0, max_batch: int = 0,
-                 device: int = 0):
-        cfg = SlakServerConfig(scheme, device, max_batch, 0, memory_budget_bytes)
+                 device: int = 0, slots: int = 0):
+        cfg = SlakServerConfig(scheme, device, max_batch, slots, memory_budget_bytes)
         h = ctypes.c_void_p()
         check(lib().slak_server_start(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

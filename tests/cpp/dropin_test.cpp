@@ -88,8 +88,9 @@ int main() {
   for (size_t v = 0; v < rmesh.nverts; ++v) CHECK(verts[4 * v + 3] == 1.0f);
   std::printf("ok LAK: nf=%zu nv=%zu\n", rmesh.nfaces, rmesh.nverts);
 
+  SMesh scmesh = make_torus<SMesh>(a, b);  // CatmullClark has a MeshInfo type of its own
   SMesh smesh;
-  CatmullClark::subdivideIteratively(cmesh, smesh, levels);
+  CatmullClark::subdivideIteratively(scmesh, smesh, levels);
   CHECK(smesh.nfaces == nf && smesh.is_reduced && smesh.ptr == nullptr && smesh.vals == nullptr);
   auto sverts = download(smesh.verts, 4 * smesh.nverts);
   // same refined positions up to the edge-point permutation: the control
@@ -115,6 +116,7 @@ int main() {
 
   rmesh.freeAndReset();
   smesh.freeAndReset();
+  scmesh.freeAndReset();
   cmesh.freeAndReset();
   CHECK(rmesh.ids == nullptr && cmesh.verts == nullptr);
   std::printf("ok dropin\n");
