@@ -1,14 +1,18 @@
 #!/usr/bin/env python3
 """Refined faces/s of SLAK's Catmull-Clark chain on one MI355X.
 
-Default workload: the 256 x 128 mixed-polygon torus (quads, triangles with
-p = 0.25, one pentagon per grid column), 4 levels: one poly level then three
-quad levels, ~9.4 M refined faces. A step is one full
-LAKCatmullClark::subdivideIteratively call through the C-ABI on the
-device-resident control cage (output allocation and free included). Prints
-one JSON line with the throughput, the roofline of the dominant kernel
-(HIP events inside the library on the launch stream) and the CPU baseline
-(the same LAK algorithm, FP32, on the host's cores).
+Headline workload: the batched uneven workload, 4096 independent seeded tori
+(grid dims uniform in [4, 64]^2, 20 % of quads triangulated, depth uniform in
+{1..4}, about 443 M final faces). A step is one replay of the whole batched
+level chain (one launch per kernel per level over the union of the meshes
+still active at that level, captured once as a hipGraph) with the control
+cages resident on the device. Prints one JSON line with the throughput, the
+roofline of the dominant kernel (HIP events inside the library on the launch
+stream, HBM counters from rocprofv3), the plain per-mesh form for the
+speedup, secondary legs (the 256 x 128 mixed torus at 4 levels, the
+1024 x 1024 torus at 2 levels with per-stage GB/s, the served stream) and
+the CPU baseline (the same LAK algorithm, FP32, on the host's cores) with the
+per-config position error against the FP64 oracle.
 """
 from __future__ import annotations
 
@@ -26,14 +30,17 @@ if ROOT not in sys.path:
 METRIC = "refined faces/s + HBM GB/s; batched-vs-plain speedup; serve p50/p99 latency"
 HBM_PEAK_GBPS = 8000.0
 
-WORKLOADS = {
+SINGLE = {
     "cfg1": ("32x16 closed quad torus, 3 levels", 3),
     "cfg2": ("256x128 mixed-polygon torus (p_tri=0.25, 1 pentagon per column), 4 levels", 4),
     "cfg3": ("1024x1024 closed quad torus, 2 levels", 2),
 }
+CFG4_DESC = "4096 seeded tori, dims U[4,64]^2, 20% of quads triangulated, depth U{1..4}, batched hipGraph replay"
+# the stage and kernel that dominate each workload's time
+CFG4_STAGE = "refine_vertex_final_quad"
 
 
-def make_mesh(name):
+def make_single(name):
     from slak_amd import meshgen
 
     return {"cfg1": meshgen.config1, "cfg2": meshgen.config2, "cfg3": meshgen.config3}[name]()
@@ -49,16 +56,53 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def time_gpu(mesh, levels, scheme, steps, warmup, profile_steps=3):
-    from slak_amd import capi
-    from slak_amd.mesh import MeshInfo, _subdivide
+def stage_roofline(prof, stage=None):
+    """Roofline of one stage of a library profile (the slowest one by
+    default): its algorithmic bytes per launch over its mean HIP-event launch
+    duration, against the 8 TB/s HBM peak; plus the whole chain's figure."""
+    stages = prof["stages"]
+    total_ms = sum(s["total_ms"] for s in stages)
+    total_b = sum(s["bytes"] for s in stages)
+    dom = max(stages, key=lambda s: s["total_ms"]) if stage is None else next(s for s in stages if s["name"] == stage)
+    ms_per_launch = dom["total_ms"] / dom["launches"]
+    bytes_per_launch = dom["bytes"] / dom["launches"]
+    achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
+    return {
+        "bound": "hbm",
+        "kernel": dom["name"],
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 4),
+        "traffic": None,
+        "bytes_per_launch": int(bytes_per_launch),
+        "us_per_launch": round(ms_per_launch * 1e3, 2),
+        "launches": int(dom["launches"]),
+        "chain_gbps": round(total_b / (total_ms * 1e-3) / 1e9, 1),
+        "chain_frac": round(total_b / (total_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+    }
 
+
+def stage_table(prof):
+    return [{"name": s["name"], "launches": s["launches"], "us": round(s["total_ms"] * 1e3, 1),
+             "bytes": int(s["bytes"]), "gbps": round(s["bytes"] / (s["total_ms"] * 1e-3) / 1e9, 1)}
+            for s in prof["stages"]]
+
+
+# ---------------------------------------------------------------------------
+# GPU legs
+
+
+def time_single(mesh, levels, scheme, steps, warmup, profile_steps=3):
+    """K full subdivideIteratively calls through the C-ABI on the
+    device-resident control cage (output allocation and free included)."""
     import ctypes
+
+    from slak_amd import capi
+    from slak_amd.mesh import MeshInfo
+
     c = MeshInfo.from_host(mesh)
-    faces = 0
     try:
-        # one library call per step (output allocation and free included),
-        # with the ctypes structures built once
         lib = capi.lib()
         cin = c.to_c()
         cout = capi.SlakMesh()
@@ -75,14 +119,12 @@ def time_gpu(mesh, levels, scheme, steps, warmup, profile_steps=3):
 
         for _ in range(warmup):
             step()
-        capi.check(capi.lib().slak_device_synchronize())
+        capi.check(lib.slak_device_synchronize())
         t0 = time.perf_counter()
         for _ in range(steps):
             st = step()
-        capi.check(capi.lib().slak_device_synchronize())
-        t1 = time.perf_counter()
-        faces = int(st.out_faces)
-        # per-stage HIP-event profile over a few extra steps
+        capi.check(lib.slak_device_synchronize())
+        secs = time.perf_counter() - t0
         prof = None
         if profile_steps:
             capi.profile_enable(True)
@@ -90,78 +132,34 @@ def time_gpu(mesh, levels, scheme, steps, warmup, profile_steps=3):
                 step()
             prof = capi.profile_report()
             capi.profile_enable(False)
-        return (t1 - t0), faces, st, prof
+        return secs, int(st.out_faces), st, prof
     finally:
         c.freeAndReset()
 
 
-def roofline(prof):
-    stages = prof["stages"]
-    total_ms = sum(s["total_ms"] for s in stages)
-    total_b = sum(s["bytes"] for s in stages)
-    dom = max(stages, key=lambda s: s["total_ms"])
-    ms_per_launch = dom["total_ms"] / dom["launches"]
-    bytes_per_launch = dom["bytes"] / dom["launches"]
-    achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
-    return {
-        "bound": "hbm",
-        "kernel": dom["name"],
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBPS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBPS, 4),
-        "traffic": None,
-        "bytes_per_launch": int(bytes_per_launch),
-        "us_per_launch": round(ms_per_launch * 1e3, 2),
-        "chain_gbps": round(total_b / (total_ms * 1e-3) / 1e9, 1),
-        "chain_frac": round(total_b / (total_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-    }
+def fetch_result(m):
+    import numpy as np
+
+    from slak_amd.mesh import to_host
+    ids = to_host(m.ids, m.nnz, np.int32)
+    verts = to_host(m.verts, 4 * m.nverts, np.float32).reshape(-1, 4)
+    ptr = to_host(m.ptr, m.nfaces + 1, np.int32) if m.ptr else None
+    vals = to_host(m.vals, m.nnz, np.int32) if m.vals else None
+    return ids, verts, ptr, vals
 
 
-def cpu_baseline(workload, budget_s):
-    """The oracle's FP32 LAK algorithm on the host (a bounded sample: the same
-    mesh with one level fewer), single-threaded and OpenMP."""
-    from oracle import pyoracle
-    from slak_amd import meshgen
-
-    if not os.path.exists(pyoracle.LIB_PATH):
-        pyoracle.build()
-    mesh = make_mesh(workload)
-    levels = max(1, WORKLOADS[workload][1] - 1)
-    threads = min(16, os.cpu_count() or 1)
-    ms1, faces = pyoracle.time_cpu(mesh, levels, 1, warmup=0, reps=1)
-    reps1 = max(1, min(5, int(budget_s * 0.5 / max(ms1 * 1e-3, 1e-3))))
-    if reps1 > 1:
-        ms1, faces = pyoracle.time_cpu(mesh, levels, 1, warmup=1, reps=reps1)
-    msn, _ = pyoracle.time_cpu(mesh, levels, threads, warmup=1, reps=5)
-    return {
-        "value": round(faces / (msn * 1e-3), 1),
-        "unit": "faces/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"{workload} mesh refined {levels} levels ({faces} faces), FP32 LAK, median of runs after 1 warm-up",
-        "value_1thread": round(faces / (ms1 * 1e-3), 1),
-        "cpu_model": cpu_model(),
-        "host_cores": os.cpu_count(),
-    }
-
-
-def batched_vs_plain(n_meshes, steps, scheme, seed=4):
-    """Batched uneven workload: n_meshes seeded tori (dims uniform in [4,64]^2,
-    20% of quads triangulated, depth uniform in {1..4}). Plain = one
-    subdivideIteratively call per mesh; batched = one launch per kernel per
-    level over the union, replayed as a hipGraph. Outputs are identical."""
-    from slak_amd import capi, meshgen
+def batched_leg(spec, scheme, steps, warmup, keep_results=64):
+    """cfg4: the plain form (one subdivideIteratively per mesh) and the
+    batched chain (graph replays). Returns the headline numbers, the profile
+    of one eager run and the first keep_results meshes' outputs (the parity
+    sample)."""
+    from slak_amd import capi
     from slak_amd.batch import Batch
     from slak_amd.mesh import MeshInfo, _subdivide
 
-    t_gen = time.perf_counter()
-    spec = meshgen.config4(n_meshes=n_meshes, seed=seed)
-    gen_s = time.perf_counter() - t_gen
     dev = [MeshInfo.from_host(h) for h in spec.meshes]
     try:
-        # plain: per-mesh calls (one warm-up pass over a slice first)
-        for d, lv in list(zip(dev, spec.levels))[:64]:
+        for d, lv in list(zip(dev, spec.levels))[:64]:  # warm the per-mesh path
             r = MeshInfo()
             _subdivide(scheme, d, r, lv)
             r.freeAndReset()
@@ -175,50 +173,46 @@ def batched_vs_plain(n_meshes, steps, scheme, seed=4):
             r.freeAndReset()
         capi.check(capi.lib().slak_device_synchronize())
         plain_s = time.perf_counter() - t0
-        # batched: create (validation + first-level analysis + allocation), graph replays
+
         t0 = time.perf_counter()
         b = Batch(dev, spec.levels, scheme)
         create_s = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        b.run(use_graph=True)  # first run captures the graph
-        first_s = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            b.run(use_graph=True)
-        batched_s = (time.perf_counter() - t0) / steps
-        inf = b.info()
-        assert int(inf.out_faces) == faces
-        capi.profile_enable(True)
-        b.run(use_graph=False)
-        prof = capi.profile_report()
-        capi.profile_enable(False)
-        b.close()
+        try:
+            t0 = time.perf_counter()
+            b.run(use_graph=True)  # capture + first replay
+            first_s = time.perf_counter() - t0
+            for _ in range(warmup):
+                b.run(use_graph=True)
+            capi.check(capi.lib().slak_device_synchronize())
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                b.run(use_graph=True)
+            capi.check(capi.lib().slak_device_synchronize())
+            secs = time.perf_counter() - t0
+            inf = b.info()
+            assert int(inf.out_faces) == faces, "batched and per-mesh face counts differ"
+            capi.profile_enable(True)
+            b.run(use_graph=False)
+            prof = capi.profile_report()
+            capi.profile_enable(False)
+            kept = [fetch_result(b.result(i)) for i in range(min(keep_results, len(dev)))]
+        finally:
+            b.close()
     finally:
         for d in dev:
             d.freeAndReset()
-    rl = roofline(prof)
     return {
-        "workload": f"{n_meshes} seeded tori, dims U[4,64]^2, 20% triangulated, depth U{{1..4}}",
-        "out_faces": faces,
-        "plain_per_mesh_s": round(plain_s, 4),
-        "batched_graph_s": round(batched_s, 5),
-        "batched_create_s": round(create_s, 4),
-        "batched_first_run_s": round(first_s, 4),
-        "speedup_vs_plain": round(plain_s / batched_s, 2),
-        "speedup_vs_plain_incl_create": round(plain_s / (create_s + first_s), 2),
-        "faces_per_s_batched": round(faces / batched_s, 1),
-        "faces_per_s_plain": round(faces / plain_s, 1),
-        "roofline": rl,
-        "stages": prof["stages"],
-        "generation_s": round(gen_s, 2),
+        "faces": faces, "secs": secs, "plain_s": plain_s, "create_s": create_s, "first_s": first_s,
+        "device_bytes": int(inf.device_bytes), "host_syncs": int(inf.host_syncs), "prof": prof, "kept": kept,
     }
 
 
-def serve_poisson(n_requests, qps_fraction, scheme, seed=5):
-    """Served stream: Poisson arrivals of requests (torus, faces log-uniform in
+def serve_leg(n_requests, qps_fraction, scheme, seed=5, keep=()):
+    """cfg5: Poisson arrivals of requests (torus, faces log-uniform in
     [100, 1e6], 10% triangles, level uniform in {1..3}); latency from submit
     until the refined mesh is resident. QPS is stated as a fraction of the
-    saturation rate measured with every request submitted at once."""
+    saturation rate measured with every request submitted at once. The
+    results of the requests in `keep` are copied back (parity sample)."""
     import numpy as np
 
     from slak_amd import meshgen
@@ -227,8 +221,9 @@ def serve_poisson(n_requests, qps_fraction, scheme, seed=5):
 
     reqs = [meshgen.request_mesh(seed, i) for i in range(n_requests)]
     dev = [MeshInfo.from_host(m) for m, _ in reqs]
+    kept = {}
     try:
-        def run(qps):
+        def run(qps, collect):
             rng = np.random.default_rng(seed)
             gaps = rng.exponential(1.0 / qps, size=n_requests) if qps else np.zeros(n_requests)
             lat, faces = [], 0
@@ -236,16 +231,18 @@ def serve_poisson(n_requests, qps_fraction, scheme, seed=5):
                 t0 = time.perf_counter()
                 ids = []
                 t_next = t0
-                for i, (d, (m, lv)) in enumerate(zip(dev, reqs)):
+                for i, d in enumerate(dev):
                     t_next += gaps[i]
                     now = time.perf_counter()
                     if t_next > now:
                         time.sleep(t_next - now)
-                    ids.append(srv.submit(d, lv))
-                for rid in ids:
+                    ids.append(srv.submit(d, reqs[i][1]))
+                for i, rid in enumerate(ids):
                     r, st = srv.wait(rid)
                     lat.append(st.latency_us / 1e3)
                     faces += int(st.out_faces)
+                    if collect and i in keep:
+                        kept[i] = fetch_result(r)
                     r.freeAndReset()
                 wall = time.perf_counter() - t0
                 sst = srv.stats()
@@ -261,10 +258,11 @@ def serve_poisson(n_requests, qps_fraction, scheme, seed=5):
                 "out_faces": faces,
             }
 
-        sat = run(0.0)  # everything at once: saturation throughput
+        run(0.0, False)  # warm-up
+        sat = run(0.0, False)
         sat_qps = n_requests / sat["wall_s"]
         qps = qps_fraction * sat_qps
-        loaded = run(qps)
+        loaded = run(qps, True)
     finally:
         for d in dev:
             d.freeAndReset()
@@ -275,38 +273,65 @@ def serve_poisson(n_requests, qps_fraction, scheme, seed=5):
         "saturation_qps": round(sat_qps, 2),
         **loaded,
         "saturation": sat,
-    }
+    }, reqs, kept
 
 
-# profiler stage -> rocprof kernel name prefix ({s}: scheme template argument)
-# stage -> pattern of its kernel's name (re.match; {s}: 0 LAK, 1 SLAK)
+# ---------------------------------------------------------------------------
+# HBM counters (rocprofv3 in child processes, before this process touches the GPU)
+
+# kernel of each roofline stage ({s}: 0 LAK, 1 SLAK; multi: the table path)
 STAGE_KERNELS = {
-    "refine_vertex_final_quad": r"k_final<{s}, \w+, \w+, true>",
-    "refine_vertex_pos_quad": r"k_final<{s}, \w+, \w+, false>",
+    "refine_vertex_final_quad": r"k_final<{s}, \w+, {multi}, true>",
+    "refine_vertex_pos_quad": r"k_final<{s}, \w+, {multi}, false>",
     "refine_vertex_quad": r"k_level<true, {s},",
     "refine_vertex_poly": r"k_level<false, {s},",
     "facepoints_quad": r"k_faces<true, true, false>",
     "facepoints_poly": r"k_faces<false, true, false>",
+    "gather": r"k_gather",
+    "count_quad": r"k_faces<true, false, true>",
+    "count_poly": r"k_faces<false, false, true>",
+    "transpose_quad": r"k_transpose<true>",
+    "transpose_poly": r"k_transpose<false>",
+    "spgemm_symbolic_quad": r"k_symbolic<true, {s},",
+    "spgemm_symbolic_poly": r"k_symbolic<false, {s},",
+    "number_facepoints_quad": r"k_number_faces<true, {s},",
+    "number_facepoints_poly": r"k_number_faces<false, {s},",
 }
+PMC_WORKLOADS = ("cfg4", "cfg2", "cfg3")
 
 
-def pmc_child(workload, scheme_name):
+def pmc_child(scheme_name, n_meshes):
     """The profiled program of hbm_traffic(): the 1 GiB copy probe (the
-    counters' calibration), then one warm-up and one timed chain."""
+    counters' calibration), then each workload of PMC_WORKLOADS (one warm-up
+    and one measured run), separated by a 64 KiB copy probe as a marker."""
     import ctypes
-    from slak_amd import capi
+
+    from slak_amd import capi, meshgen
+    from slak_amd.batch import Batch
+    from slak_amd.mesh import MeshInfo
     gbps = ctypes.c_double()
-    capi.check(capi.lib().slak_copy_probe(1 << 30, 1, ctypes.byref(gbps)))
     scheme = capi.SLAK_SCHEME_LAK if scheme_name == "lak" else capi.SLAK_SCHEME_SLAK
-    time_gpu(make_mesh(workload), WORKLOADS[workload][1], scheme, 1, 1, profile_steps=0)
+    capi.check(capi.lib().slak_copy_probe(1 << 30, 1, ctypes.byref(gbps)))
+    for w in PMC_WORKLOADS:
+        capi.check(capi.lib().slak_copy_probe(1 << 16, 1, ctypes.byref(gbps)))  # marker
+        if w == "cfg4":
+            spec = meshgen.config4(n_meshes=n_meshes)
+            dev = [MeshInfo.from_host(h) for h in spec.meshes]
+            with Batch(dev, spec.levels, scheme) as b:
+                b.run(use_graph=False)
+                b.run(use_graph=False)
+            for d in dev:
+                d.freeAndReset()
+        else:
+            time_single(make_single(w), SINGLE[w][1], scheme, 1, 1, profile_steps=0)
+        print(f"pmc child: {w} done", flush=True)
 
 
-def hbm_traffic(workload, scheme_name):
-    """HBM bytes per dispatch of every kernel of the chain, from two
+def hbm_traffic(scheme_name, n_meshes):
+    """HBM bytes per dispatch of every kernel of each workload, from two
     rocprofv3 --pmc runs of pmc_child (FETCH_SIZE, then WRITE_SIZE), each
-    scaled by what the counter reads for the copy probe (1 GiB read, 1 GiB
-    written). Runs before this process touches the GPU; None when rocprofv3
-    is unavailable or a run fails."""
+    scaled by what the counter reads for the 1 GiB copy probe (1 GiB read,
+    1 GiB written). None when rocprofv3 is unavailable or a run fails."""
     import csv
     import glob
     import shutil
@@ -319,65 +344,95 @@ def hbm_traffic(workload, scheme_name):
         return None
     out = tempfile.mkdtemp(prefix="slak_pmc_")
     counts = {}
+    calib = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", counter.lower(), "--",
-               sys.executable, os.path.abspath(__file__), "--pmc-child", "--workload", workload,
-               "--scheme", scheme_name]
+               sys.executable, os.path.abspath(__file__), "--pmc-child", "--scheme", scheme_name,
+               "--batch-meshes", str(n_meshes)]
         try:
-            r = subprocess.run(cmd, capture_output=True, text=True, timeout=180, env=dict(os.environ))
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=300, env=dict(os.environ))
         except subprocess.TimeoutExpired:
             return None
         if r.returncode != 0:
             return None
-        per = defaultdict(list)
+        rows = []
         for f in glob.glob(os.path.join(out, "**", f"{counter.lower()}_counter_collection.csv"), recursive=True):
             with open(f) as fh:
-                for row in csv.DictReader(fh):
-                    if row.get("Counter_Name") == counter:
-                        name = row["Kernel_Name"]
-                        for pre in ("void ", "slak::(anonymous namespace)::", "slak::"):
-                            name = name.replace(pre, "")
-                        per[name.split("(")[0]].append(float(row["Counter_Value"]) * 1024.0)
+                rows += [row for row in csv.DictReader(fh) if row.get("Counter_Name") == counter]
+        rows.sort(key=lambda row: int(row["Dispatch_Id"]))
+        per = {w: defaultdict(list) for w in PMC_WORKLOADS}
+        # the 1 GiB copies (a warm-up and the measured one) calibrate; each
+        # run of 64 KiB marker copies (two dispatches) opens the next workload
+        seg, in_marker = -1, False
+        for row in rows:
+            name = row["Kernel_Name"]
+            for pre in ("void ", "slak::(anonymous namespace)::", "slak::"):
+                name = name.replace(pre, "")
+            name = name.split("(")[0]
+            value = float(row["Counter_Value"]) * 1024.0
+            if name == "k_copy_probe":
+                if int(row.get("Grid_Size", 0)) == (1 << 30) // 16:
+                    calib[counter] = value
+                elif not in_marker:
+                    seg += 1
+                    in_marker = True
+                continue
+            in_marker = False
+            if 0 <= seg < len(PMC_WORKLOADS):
+                per[PMC_WORKLOADS[seg]][name].append(value)
         counts[counter] = per
     shutil.rmtree(out, ignore_errors=True)
-    fe, wr = counts["FETCH_SIZE"], counts["WRITE_SIZE"]
-    if not fe.get("k_copy_probe") or not wr.get("k_copy_probe"):
+    if "FETCH_SIZE" not in calib or "WRITE_SIZE" not in calib:
         return None
     copy = float(1 << 30)
-    cal_f = copy / (sum(fe["k_copy_probe"]) / len(fe["k_copy_probe"]))
-    cal_w = copy / (sum(wr["k_copy_probe"]) / len(wr["k_copy_probe"]))
-    kernels = {}
-    for name in set(fe) | set(wr):
-        f, w = fe.get(name, []), wr.get(name, [])
-        kernels[name] = {"dispatches": max(len(f), len(w)),
-                         "read": cal_f * sum(f) / max(len(f), 1), "write": cal_w * sum(w) / max(len(w), 1)}
-    return {"calibration": {"fetch": round(cal_f, 4), "write": round(cal_w, 4)}, "kernels": kernels}
+    cal_f, cal_w = copy / calib["FETCH_SIZE"], copy / calib["WRITE_SIZE"]
+    result = {"calibration": {"fetch": round(cal_f, 4), "write": round(cal_w, 4)}, "workloads": {}}
+    for w in PMC_WORKLOADS:
+        fe, wr = counts["FETCH_SIZE"][w], counts["WRITE_SIZE"][w]
+        kernels = {}
+        for name in set(fe) | set(wr):
+            f, wv = fe.get(name, []), wr.get(name, [])
+            kernels[name] = {"dispatches": max(len(f), len(wv)),
+                             "read": cal_f * sum(f) / max(len(f), 1), "write": cal_w * sum(wv) / max(len(wv), 1)}
+        result["workloads"][w] = kernels
+    return result
 
 
-def add_traffic(roof, traffic, scheme_name):
-    """roofline.traffic: average calibrated HBM bytes per dispatch of the
-    dominant stage's kernels (over the same levels as bytes_per_launch)."""
-    if not traffic or roof["kernel"] not in STAGE_KERNELS:
-        return roof
+def stage_traffic(traffic, workload, stage, scheme_name, multi):
+    """Average calibrated HBM bytes (read, write) per dispatch of a stage's kernel."""
     import re
 
-    pattern = re.compile(STAGE_KERNELS[roof["kernel"]].format(s=0 if scheme_name == "lak" else 1))
-    sel = [v for k, v in traffic["kernels"].items() if pattern.match(k)]
+    if not traffic or stage not in STAGE_KERNELS:
+        return None
+    kernels = traffic["workloads"].get(workload, {})
+    pat = re.compile(STAGE_KERNELS[stage].format(s=0 if scheme_name == "lak" else 1,
+                                                  multi="true" if multi else "false"))
+    sel = [v for k, v in kernels.items() if pat.match(k)]
     n = sum(v["dispatches"] for v in sel)
     if not n:
-        return roof
+        return None
     rd = sum(v["read"] * v["dispatches"] for v in sel) / n
     wt = sum(v["write"] * v["dispatches"] for v in sel) / n
-    roof["traffic"] = int(rd + wt)
-    roof["traffic_read"] = int(rd)
-    roof["traffic_write"] = int(wt)
-    roof["traffic_calibration"] = traffic["calibration"]
-    roof["traffic_source"] = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the 1 GiB copy probe"
+    return int(rd), int(wt)
+
+
+def add_traffic(roof, traffic, workload, scheme_name, multi):
+    t = stage_traffic(traffic, workload, roof["kernel"], scheme_name, multi)
+    if t:
+        roof["traffic"] = t[0] + t[1]
+        roof["traffic_read"], roof["traffic_write"] = t
+        roof["traffic_over_algorithmic"] = round((t[0] + t[1]) / roof["bytes_per_launch"], 3)
+        roof["traffic_calibration"] = traffic["calibration"]
+        roof["traffic_source"] = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the 1 GiB copy probe"
     return roof
 
 
+# ---------------------------------------------------------------------------
+# I/O floor
+
+
 def io_floor_bytes(mesh, levels, lak_form):
-    """Implementation-independent I/O floor of the chain: per level, read the
+    """Implementation-independent I/O floor of a chain: per level, read the
     level's ids (+ ptr + vals in LAK form) and positions, write the refined
     ids (+ ptr + vals) and positions; summed over the levels. Sizes follow
     the closed forms (n_f' = nnz, nnz' = 4 nnz, n_e = (nnz - n_ext)/2 + n_ext,
@@ -388,8 +443,9 @@ def io_floor_bytes(mesh, levels, lak_form):
     nxt = np.arange(len(ids)) + 1
     nxt[ptr[1:] - 1] = ptr[:-1]  # cyclic successor within each face
     a, b = ids, ids[nxt]
-    he = set(zip(a.tolist(), b.tolist()))
-    n_ext = sum(1 for (x, y) in he if (y, x) not in he)
+    key = a * (int(mesh.nverts) + 1) + b
+    rev = b * (int(mesh.nverts) + 1) + a
+    n_ext = int((~np.isin(rev, key)).sum())
     nv, nf, nnz = int(mesh.nverts), int(mesh.nfaces), int(len(ids))
     total = 0.0
     for _ in range(levels):
@@ -402,11 +458,103 @@ def io_floor_bytes(mesh, levels, lak_form):
     return total
 
 
-def io_floor(mesh, levels, lak_form, secs_per_step):
-    b = io_floor_bytes(mesh, levels, lak_form)
+def io_floor(b, secs_per_step):
     gbps = b / secs_per_step / 1e9
-    return {"bytes_per_chain": int(b), "gbps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4),
+    return {"bytes_per_step": int(b), "gbps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4),
             "note": "per level: read ids(+ptr+vals)+positions, write refined ids(+ptr+vals)+positions"}
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline and the per-config position error (the FP64 oracle as checker)
+
+
+def cpu_baseline(spec, scheme, budget_s):
+    """The oracle's FP32 LAK algorithm on the host: the headline workload
+    (cfg4, every mesh at its own depth) with the meshes spread over all host
+    threads, and single-threaded on a bounded prefix of the same batch."""
+    from oracle import pyoracle
+
+    if not os.path.exists(pyoracle.LIB_PATH):
+        pyoracle.build()
+    threads = pyoracle.max_threads()
+    msn, faces = pyoracle.time_cpu_batch(spec.meshes, spec.levels, threads, scheme=scheme, warmup=0, reps=1)
+    # single thread: the prefix of the batch that fits about half the budget at the OpenMP rate / threads
+    per_face_1 = msn * 1e-3 * threads / max(faces, 1)
+    want = max(1.0, 0.5 * budget_s / per_face_1)
+    n1, acc = 0, 0
+    for h, lv in zip(spec.meshes, spec.levels):
+        acc += h.nnz * 4 ** (max(lv, 1) - 1)  # final faces of the mesh
+        n1 += 1
+        if acc >= want:
+            break
+    ms1, faces1 = pyoracle.time_cpu_batch(spec.meshes[:n1], spec.levels[:n1], 1, scheme=scheme, warmup=0, reps=1)
+    return {
+        "value": round(faces / (msn * 1e-3), 1),
+        "unit": "faces/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"the full cfg4 batch ({len(spec.meshes)} meshes, {faces} final faces, each mesh at its own depth), "
+                  f"FP32 LAK, meshes spread over {threads} OpenMP threads, one timed pass",
+        "value_1thread": round(faces1 / (ms1 * 1e-3), 1),
+        "sample_1thread": f"the first {n1} meshes of the same batch ({faces1} final faces), one thread",
+        "cpu_model": cpu_model(),
+        "host_cores": os.cpu_count(),
+    }
+
+
+def position_error(ref_P, verts, diag):
+    import numpy as np
+    return float(np.abs(verts.astype(np.float64) - ref_P).max()) / diag
+
+
+def parity_check(oracle, host, levels, scheme, got):
+    """Topology bit-exact and max |p - p_fp64| / bbox diagonal of one GPU result."""
+    import numpy as np
+    ids, verts, ptr, vals = got
+    ref = oracle.subdivide(host, levels, scheme=scheme, precision=64, nthreads=oracle.max_threads())
+    exact = len(ids) == len(ref.ids) and bool(np.array_equal(ids, ref.ids))
+    if scheme == 0:
+        exact = exact and ptr is not None and bool(np.array_equal(ptr, ref.ptr)) and bool(np.array_equal(vals, ref.vals))
+    err = position_error(ref.P, verts, host.bbox_diag()) if verts.shape[0] == ref.nv else float("inf")
+    return exact, err
+
+
+def parity_per_config(scheme, cfg4_spec, cfg4_kept, cfg5_reqs, cfg5_kept):
+    """The GPU result of every configuration against the FP64 oracle: cfg1-3
+    refined again here, the first meshes of the cfg4 batch and the kept cfg5
+    requests from their legs."""
+    from oracle import pyoracle
+    from slak_amd.mesh import subdivide_host
+
+    out = {}
+    for w, (_, levels) in SINGLE.items():
+        m = make_single(w)
+        r, _ = subdivide_host(m, levels, scheme)
+        exact, err = parity_check(pyoracle, m, levels, scheme, (r.ids, r.verts, r.ptr, r.vals))
+        out[w] = {"topology_bit_exact": exact, "max_err_over_diag": err, "sample": "full mesh"}
+    if cfg4_kept:
+        errs, ok = [], True
+        for i, got in enumerate(cfg4_kept):
+            e, err = parity_check(pyoracle, cfg4_spec.meshes[i], cfg4_spec.levels[i], scheme, got)
+            ok &= e
+            errs.append(err)
+        out["cfg4"] = {"topology_bit_exact": ok, "max_err_over_diag": max(errs),
+                       "sample": f"the first {len(cfg4_kept)} meshes of the batch, from the graph replay"}
+    if cfg5_kept:
+        errs, ok = [], True
+        for i, got in sorted(cfg5_kept.items()):
+            m, lv = cfg5_reqs[i]
+            e, err = parity_check(pyoracle, m, lv, scheme, got)
+            ok &= e
+            errs.append(err)
+        out["cfg5"] = {"topology_bit_exact": ok, "max_err_over_diag": max(errs),
+                       "sample": f"served requests {sorted(cfg5_kept)} of the Poisson run"}
+    for v in out.values():
+        v["max_err_over_diag"] = float(f"{v['max_err_over_diag']:.3e}")
+    return out
+
+
+# ---------------------------------------------------------------------------
 
 
 def main():
@@ -414,12 +562,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
     ap.add_argument("--scheme", default="lak", choices=["lak", "slak"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--batch-meshes", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle parity check")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of single-thread CPU baseline work")
+    ap.add_argument("--no-legs", action="store_true", help="skip the cfg2 / cfg3 single-mesh legs")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-qps-fraction", type=float, default=0.5)
@@ -427,7 +574,7 @@ def main():
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
-        pmc_child(args.workload, args.scheme)
+        pmc_child(args.scheme, args.batch_meshes)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (meshes are independent and would be "
@@ -436,20 +583,24 @@ def main():
         sys.exit(2)
 
     # HBM counters first, in child processes, before this one touches the GPU
-    traffic = None if args.no_traffic else hbm_traffic(args.workload, args.scheme)
+    traffic = None if args.no_traffic else hbm_traffic(args.scheme, args.batch_meshes)
 
-    from slak_amd import capi
-
-    desc, levels = WORKLOADS[args.workload]
-    mesh = make_mesh(args.workload)
-    scheme = capi.SLAK_SCHEME_LAK if args.scheme == "lak" else capi.SLAK_SCHEME_SLAK
     import ctypes
+
+    from slak_amd import capi, meshgen
+
+    scheme = capi.SLAK_SCHEME_LAK if args.scheme == "lak" else capi.SLAK_SCHEME_SLAK
+    lak_form = scheme == capi.SLAK_SCHEME_LAK
     copy_gbps = ctypes.c_double()
     capi.check(capi.lib().slak_copy_probe(1 << 30, 5, ctypes.byref(copy_gbps)))
-    secs, faces, st, prof = time_gpu(mesh, levels, scheme, args.steps, args.warmup)
-    other = capi.SLAK_SCHEME_SLAK if scheme == capi.SLAK_SCHEME_LAK else capi.SLAK_SCHEME_LAK
-    secs_o, _, _, _ = time_gpu(mesh, levels, other, max(3, args.steps // 2), 1)
-    value = faces * args.steps / secs
+
+    # headline: the 4096-mesh batch, graph replays
+    spec = meshgen.config4(n_meshes=args.batch_meshes)
+    B = batched_leg(spec, scheme, args.steps, args.warmup)
+    secs = B["secs"]
+    value = B["faces"] * args.steps / secs
+    roof = add_traffic(stage_roofline(B["prof"], CFG4_STAGE), traffic, "cfg4", args.scheme, True)
+    floor_b = sum(io_floor_bytes(h, lv, lak_form) for h, lv in zip(spec.meshes, spec.levels))
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -464,30 +615,68 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded generator, slak_amd.meshgen)",
         "config": {
-            "workload": f"{args.workload}: {desc}",
+            "workload": f"cfg4: {CFG4_DESC}",
             "scheme": args.scheme.upper(),
-            "levels": levels,
-            "input_faces": mesh.nfaces,
-            "input_verts": mesh.nverts,
-            "output_faces": faces,
-            "output_verts": int(st.out_verts),
-            "mesh_digest": mesh.digest(),
+            "meshes": len(spec.meshes),
+            "levels": "per mesh, U{1..4}",
+            "input_faces": int(sum(h.nfaces for h in spec.meshes)),
+            "output_faces": B["faces"],
             "parallelism": "single GPU",
         },
-        "roofline": add_traffic(roofline(prof), traffic, args.scheme),
-        "io_floor": io_floor(mesh, levels, args.scheme == "lak", secs / args.steps),
+        "roofline": roof,
+        "io_floor": io_floor(floor_b, secs / args.steps),
         "copy_kernel_gbps": round(copy_gbps.value, 1),
-        "other_scheme_faces_per_s": round(faces * max(3, args.steps // 2) / secs_o, 1),
-        "stages": prof["stages"],
-        "host_syncs_per_step": int(st.host_syncs),
-        "peak_device_bytes": int(st.peak_bytes),
+        "batched_vs_plain": {
+            "plain_per_mesh_s": round(B["plain_s"], 4),
+            "batched_graph_s": round(secs / args.steps, 5),
+            "speedup_vs_plain": round(B["plain_s"] / (secs / args.steps), 2),
+            "batched_create_s": round(B["create_s"], 4),
+            "batched_first_run_s": round(B["first_s"], 4),
+            "speedup_vs_plain_incl_create": round(B["plain_s"] / (B["create_s"] + B["first_s"]), 2),
+            "faces_per_s_plain": round(B["faces"] / B["plain_s"], 1),
+        },
+        "stages": stage_table(B["prof"]),
+        "device_bytes": B["device_bytes"],
     }
-    if not args.no_batch:
-        line["batched"] = batched_vs_plain(args.batch_meshes, max(3, args.steps // 4), scheme)
+
+    if not args.no_legs:
+        legs = {}
+        for w in ("cfg2", "cfg3"):
+            m = make_single(w)
+            lv = SINGLE[w][1]
+            leg = {"workload": f"{w}: {SINGLE[w][0]}"}
+            for sname, sch in (("lak", capi.SLAK_SCHEME_LAK), ("slak", capi.SLAK_SCHEME_SLAK)):
+                steps = max(3, args.steps // 2)
+                s, faces, st, prof = time_single(m, lv, sch, steps, 2)
+                r = stage_roofline(prof)
+                if sname == args.scheme:
+                    r = add_traffic(r, traffic, w, sname, False)
+                leg[sname] = {
+                    "faces_per_s": round(faces * steps / s, 1),
+                    "ms_per_chain": round(s / steps * 1e3, 4),
+                    "output_faces": faces,
+                    "roofline": r,
+                    "io_floor": io_floor(io_floor_bytes(m, lv, sname == "lak"), s / steps),
+                    "host_syncs": int(st.host_syncs),
+                }
+                if w == "cfg3":
+                    # GB/s of every SpGEMM / SpMV stage of the chain
+                    tab = stage_table(prof)
+                    for row in tab:
+                        t = stage_traffic(traffic, w, row["name"], sname, False) if sname == args.scheme else None
+                        if t:
+                            row["hbm_read"], row["hbm_write"] = t
+                    leg[sname]["stages"] = tab
+            legs[w] = leg
+        line["legs"] = legs
+
+    cfg5_reqs, cfg5_kept = None, {}
     if not args.no_serve:
-        line["serve"] = serve_poisson(args.serve_requests, args.serve_qps_fraction, scheme)
+        keep = (0, 1, 2, 4, 5, 6, 7, 8)
+        line["serve"], cfg5_reqs, cfg5_kept = serve_leg(args.serve_requests, args.serve_qps_fraction, scheme, keep=keep)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_budget)
+        line["cpu_baseline"] = cpu_baseline(spec, scheme, args.cpu_budget)
+        line["parity"] = parity_per_config(scheme, spec, B["kept"], cfg5_reqs, cfg5_kept)
     print(json.dumps(line), flush=True)
 
 

@@ -874,6 +874,8 @@ __device__ __forceinline__ float4 quad_prev(float4 x) {
                      __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.z), ctrl, 0xF, 0xF, false)),
                      __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.w), ctrl, 0xF, 0xF, false)));
 }
+__device__ __forceinline__ int32_t quad_next_i(int32_t x) { return __builtin_amdgcn_mov_dpp(x, 0x39, 0xF, 0xF, false); }
+__device__ __forceinline__ int32_t quad_prev_i(int32_t x) { return __builtin_amdgcn_mov_dpp(x, 0x93, 0xF, 0xF, false); }
 __device__ __forceinline__ float4 f4sel(bool c, float4 a, float4 b) { return c ? a : b; }
 // face point of a quad from its corners' positions in corner order (the sum
 // and division k_faces uses)
@@ -895,8 +897,13 @@ __device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg&
   const int32_t tn = q0 | ((t + 1) & 3), tp = q0 | ((t + 3) & 3);
   const int32_t* __restrict__ ids = V.ids + cb;
   const int2* __restrict__ ci = V.ci + cb;
-  const int32_t v = ids[t] - vb, w = ids[tn] - vb, u = ids[tp] - vb;
-  const int2 c1 = ci[t], c0 = ci[tp];
+  // the quad's four lanes load their own corner once; w (next corner's
+  // vertex), u and the in-half-edge's (edge, twin) come from the
+  // neighbouring lanes
+  const int32_t v = ids[t] - vb;
+  const int32_t w = quad_next_i(v), u = quad_prev_i(v);
+  const int2 c1 = ci[t];
+  const int2 c0 = make_int2(quad_prev_i(c1.x), quad_prev_i(c1.y));
   const int32_t emax = S.ne > 0 ? S.ne - 1 : 0;
   const int32_t e1 = clampi(c1.x, emax), e0 = clampi(c0.x, emax);
   // twins are union corners
@@ -940,8 +947,22 @@ __device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg&
   const float4 fpt = face_point(Pv, Pe, PF, Pu);
   out[nv1 + t] = fpt;
   const float4 fptn = quad_next(fpt);
-  const float4 fq1 = quad_lane<1>(fpt), fq2 = quad_lane<2>(fpt), fq3 = quad_lane<3>(fpt);
-  const float4 pq1 = quad_lane<1>(Pe), pq2 = quad_lane<2>(Pe), pq3 = quad_lane<3>(Pe);
+  // vertex point of fp(F): row 4t'+2 for the corners t' of F in order,
+  // in-neighbours ep(e1(t')); never on the boundary. Every lane of the quad
+  // forms the sum from the quad's lanes in corner order (each term consumed
+  // as it arrives), lane 0 stores it.
+  {
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    s = f4add(s, quad_lane<0>(fpt));
+    s = f4add(s, quad_lane<1>(fpt));
+    s = f4add(s, quad_lane<2>(fpt));
+    s = f4add(s, quad_lane<3>(fpt));
+    s = f4add(s, quad_lane<0>(Pe));
+    s = f4add(s, quad_lane<1>(Pe));
+    s = f4add(s, quad_lane<2>(Pe));
+    s = f4add(s, quad_lane<3>(Pe));
+    if ((t & 3) == 0) out[nv + F] = f4div(f4add(f4div(s, 4.0f), f4mul(PF, 2.0f)), 4.0f);
+  }
   // across e1: M1 faces s1 = (w, ep(e1), fp(G), ep(e0(s1))) and
   // next(s1) = (v, ep(e1(next(s1))), fp(G), ep(e1))
   float4 PG = make_float4(0.f, 0.f, 0.f, 0.f), fps = PG, fpsn = PG;
@@ -979,20 +1000,6 @@ __device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg&
     o[1] = make_int4(x1, eb2 + ey, fid, eb2 + ex);
     o[2] = make_int4(x2, eb2 + ez, fid, eb2 + ey);
     o[3] = make_int4(x3, eb2 + ew, fid, eb2 + ez);
-  }
-  // vertex point of fp(F): row 4t'+2 for the corners t' of F in order,
-  // in-neighbours ep(e1(t')); never on the boundary
-  if ((t & 3) == 0) {
-    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-    s = f4add(s, fpt);
-    s = f4add(s, fq1);
-    s = f4add(s, fq2);
-    s = f4add(s, fq3);
-    s = f4add(s, Pe);
-    s = f4add(s, pq1);
-    s = f4add(s, pq2);
-    s = f4add(s, pq3);
-    out[nv + F] = f4div(f4add(f4div(s, 4.0f), f4mul(PF, 2.0f)), 4.0f);
   }
   if (SCHEME == kSLAK) {
     // the M1 edge from ep(e1) to fp(F), SLAK slot nnz + next(t): every lane
@@ -1051,17 +1058,12 @@ __device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg&
     const int ra = (kb < ka) + (kc < ka) + (kd < ka);
     const int rb = (ka < kb) + (kc < kb) + (kd < kb);
     const int rc = (ka < kc) + (kb < kc) + (kd < kc);
-    float4 f[4], p[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      f[i] = f4sel(ra == i, fpt, f4sel(rb == i, fptn, f4sel(rc == i, fps, fpsn)));
-      p[i] = f4sel(ra == i, Pv, f4sel(rb == i, PF, f4sel(rc == i, Pw, PG)));
-    }
+    // the face points, then the in-neighbours, each in row order
     float4 sm = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) sm = f4add(sm, f[i]);
+    for (int i = 0; i < 4; ++i) sm = f4add(sm, f4sel(ra == i, fpt, f4sel(rb == i, fptn, f4sel(rc == i, fps, fpsn))));
 #pragma unroll
-    for (int i = 0; i < 4; ++i) sm = f4add(sm, p[i]);
+    for (int i = 0; i < 4; ++i) sm = f4add(sm, f4sel(ra == i, Pv, f4sel(rb == i, PF, f4sel(rc == i, Pw, PG))));
     q = f4div(f4add(f4div(sm, 4.0f), f4mul(Pe, 2.0f)), 4.0f);
   } else {
     q = f4add(f4add(f4mul(Pe, 0.75f), f4mul(Pw, 0.125f)), f4mul(Pv, 0.125f));
