@@ -152,6 +152,16 @@ int slak_mesh_unreduce(slak_mesh* mesh, void* stream);
  * which also writes n+1 values). Device pointers; out holds n+1 int32. */
 int slak_scan_exclusive(const int32_t* in, int32_t* out, uint64_t n, void* stream);
 
+/* slak_scan_exclusive without any allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code between
+ * allocations they control, serving loops, harnesses that forbid hipMalloc).
+ * Its flags word and tile partials live in a 1 MiB static device array per
+ * GPU, so n is limited to 268,367,872 items (SLAK_ERR_INVALID_ARGUMENT
+ * beyond). Every call on a GPU shares that array: calls that overlap on one
+ * GPU (two host threads, or two streams) race and must be serialized by the
+ * caller. Synchronous, like slak_scan_exclusive. */
+int slak_scan_exclusive_noalloc(const int32_t* in, int32_t* out, uint64_t n, void* stream);
+
 /* ---- batched subdivision (no counterpart in the reference, which refines
  * one mesh per call) ----
  * slak_batch_create validates the meshes (device pointers, kept by pointer:

@@ -129,6 +129,8 @@ EXPORTS = [
     ("slak_batch_create", ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(SlakMesh),
                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_void_p),
                                          ctypes.c_void_p]),
+    ("slak_scan_exclusive_noalloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                                   ctypes.c_void_p]),
     ("slak_digest", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
                                    ctypes.c_void_p]),
     ("slak_batch_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),

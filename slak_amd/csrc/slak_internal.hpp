@@ -73,6 +73,19 @@ void launch_sort_faces(bool by_vals, const int32_t* ptr, int32_t stride, int32_t
                        int32_t* vals, int* flags, hipStream_t st);
 void launch_fill_homogeneous(int32_t* ptr, int64_t nptr, int32_t* vals, int64_t nvals, int32_t stride, hipStream_t st);
 int64_t scan_partials_len(int64_t n);
+// Scratch of the entry points that allocate nothing: a 1 MiB file-scope
+// device array per GPU (flags word, then int64 scan partials), shared by
+// every such call on that GPU.
+constexpr size_t kNoAllocScratchBytes = size_t(1) << 20;
+struct NoAllocScratch {
+  int* flags;
+  int64_t* partial;
+  int64_t partial_len;
+};
+// the current device's scratch (its address is looked up once per device)
+NoAllocScratch noalloc_scratch();
+// largest scan the scratch holds the partials of
+int64_t noalloc_scan_max_items();
 // out[i] = sum(in[0..i)), out[n] = total (also to *total_out when given)
 void launch_scan(const int32_t* in, int32_t* out, int64_t n, int64_t* partial, int* flags, hipStream_t st,
                  int32_t* total_out = nullptr);

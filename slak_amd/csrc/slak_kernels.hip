@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <mutex>
+#include <vector>
 
 #include "slak_device.hpp"
 #include "slak_internal.hpp"
@@ -1749,6 +1750,40 @@ void launch_fill_homogeneous(int32_t* ptr, int64_t nptr, int32_t* vals, int64_t 
 }
 
 int64_t scan_partials_len(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+// ---------------------------------------------------------------------------
+// Scratch of the entry points that allocate nothing: one file-scope device
+// array (each GPU holds its own copy from module load to process exit).
+// Layout: the flags word in the first 256 bytes, then the int64 scan
+// partials. Host code uses the per-device address hipGetSymbolAddress
+// returns, never the array's name.
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+NoAllocScratch noalloc_scratch() {
+  static std::mutex mu;
+  static std::vector<void*> per_dev;
+  int dev = 0;
+  SLAK_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(mu);
+  if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1, nullptr);
+  if (!per_dev[dev]) {
+    void* p = nullptr;
+    SLAK_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    per_dev[dev] = p;
+  }
+  char* base = static_cast<char*>(per_dev[dev]);
+  NoAllocScratch s;
+  s.flags = reinterpret_cast<int*>(base);
+  s.partial = reinterpret_cast<int64_t*>(base + 256);
+  s.partial_len = (int64_t)((kNoAllocScratchBytes - 256) / sizeof(int64_t));
+  return s;
+}
+
+int64_t noalloc_scan_max_items() {
+  const int64_t partials = (int64_t)((kNoAllocScratchBytes - 256) / sizeof(int64_t));
+  return (partials - 1) * kScanTile;  // scan_partials_len(n) <= partials
+}
 
 void launch_scan(const int32_t* in, int32_t* out, int64_t n, int64_t* partial, int* flags, hipStream_t st,
                  int32_t* total_out) {

@@ -608,4 +608,22 @@ int slak_scan_exclusive(const int32_t* in, int32_t* out, uint64_t n, void* strea
   });
 }
 
+int slak_scan_exclusive_noalloc(const int32_t* in, int32_t* out, uint64_t n, void* stream) {
+  return guarded([&] {
+    if ((!in && n) || !out) throw Error(SLAK_ERR_INVALID_ARGUMENT, "null pointer");
+    const int64_t max_n = slak::noalloc_scan_max_items();
+    if (n > (uint64_t)max_n)
+      throw Error(SLAK_ERR_INVALID_ARGUMENT, "scan of " + std::to_string(n) + " items exceeds the scratch capacity of " +
+                                                 std::to_string(max_n) + " items");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const slak::NoAllocScratch s = slak::noalloc_scratch();
+    slak::launch_fill_i32(s.flags, 1, 0, st);  // the flags word starts at zero (a kernel, not a memset)
+    slak::launch_scan(in, out, (int64_t)n, s.partial, s.flags, st);
+    int fl = 0;
+    SLAK_HIP_CHECK(hipMemcpyAsync(&fl, s.flags, sizeof(int), hipMemcpyDeviceToHost, st));
+    SLAK_HIP_CHECK(hipStreamSynchronize(st));
+    if (fl) throw Error(SLAK_ERR_INVALID_ARGUMENT, "scan total exceeds int32");
+  });
+}
+
 }  // extern "C"
