@@ -91,6 +91,24 @@ __device__ __forceinline__ int seg_of(const int64_t* base, int n, int64_t x) {
   return lo;
 }
 
+// seg_of for items that rise with the lane (item = first + lane within a
+// wave): one wave-uniform binary search for the first active lane's item
+// (scalar loads), then a few steps forward per lane. Falls back to the
+// per-lane search when an item lies before the first lane's or more than 8
+// segments after it.
+__device__ __forceinline__ int seg_of_wave(const int64_t* base, int n, int64_t x) {
+  if (n <= 1) return 0;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+  const int64_t x0 = (int64_t)(((uint64_t)hi << 32) | lo);
+  if (x < x0) return seg_of(base, n, x);
+  int k = seg_of(base, n, x0);
+#pragma unroll 1
+  for (int it = 0; it < 8 && k + 1 < n && base[k + 1] <= x; ++it) ++k;
+  if (k + 1 < n && base[k + 1] <= x) k = seg_of(base, n, x);
+  return k;
+}
+
 struct LevelView {
   const int32_t* ids;
   const int32_t* fptr;   // poly levels
@@ -167,10 +185,12 @@ struct RegRow {
   int32_t fj_[CAP], tpj_[CAP];  // face / predecessor corner of the twin of out-half-edge i
   int8_t rko_[CAP], rki_[CAP];  // LAK: rank of w_i / u_j among the owned lower neighbours
 
-  __device__ __forceinline__ void load(const LevelView& L, int32_t s, int cnt) {
+  __device__ __forceinline__ void load(const LevelView& L, int32_t s, int cnt) { load(L, L.tids + s, cnt); }
+  // row: the row's unsorted corners (global memory, or the block's LDS copy)
+  __device__ __forceinline__ void load(const LevelView& L, const int32_t* row, int cnt) {
     n = cnt;
 #pragma unroll
-    for (int i = 0; i < CAP; ++i) t_[i] = (i < n) ? L.tids[s + i] : 0x7fffffff;
+    for (int i = 0; i < CAP; ++i) t_[i] = (i < n) ? row[i] : 0x7fffffff;
     // odd-even transposition network: canonical (ascending corner) order
 #pragma unroll
     for (int p = 0; p < CAP; ++p) {
@@ -188,8 +208,15 @@ struct RegRow {
         corner_geo<QUAD>(L, t_[i], f, tn, tp);
         f_[i] = f;
         tp_[i] = tp;
-        w_[i] = L.ids[tn];
-        u_[i] = L.ids[tp];
+        if (QUAD) {  // the corner's whole quad in one 16-byte load
+          const int4 q = reinterpret_cast<const int4*>(L.ids)[f];
+          const int k = t_[i] & 3;
+          w_[i] = k == 0 ? q.y : k == 1 ? q.z : k == 2 ? q.w : q.x;
+          u_[i] = k == 0 ? q.w : k == 1 ? q.x : k == 2 ? q.y : q.z;
+        } else {
+          w_[i] = L.ids[tn];
+          u_[i] = L.ids[tp];
+        }
       } else {
         f_[i] = 0;
         tp_[i] = 0;
@@ -197,6 +224,30 @@ struct RegRow {
         u_[i] = -100 - i;
       }
     }
+    pair();
+  }
+  // The same row from the symbolic pass's output: corners already sorted,
+  // each with its in-neighbour, so only the out-neighbours are gathered.
+  __device__ __forceinline__ void load_sorted(const LevelView& L, const int2* __restrict__ row, int cnt) {
+    n = cnt;
+#pragma unroll
+    for (int i = 0; i < CAP; ++i) {
+      const int2 e = (i < n) ? row[i] : make_int2(0x7fffffff, -100 - i);
+      t_[i] = e.x;
+      u_[i] = e.y;
+    }
+#pragma unroll
+    for (int i = 0; i < CAP; ++i) {
+      int32_t f = 0, tn = 0, tp = 0;
+      if (i < n) corner_geo<QUAD>(L, t_[i], f, tn, tp);
+      f_[i] = f;
+      tp_[i] = tp;
+      w_[i] = (i < n) ? L.ids[tn] : -1 - i;
+    }
+    pair();
+  }
+  // jm / im: each out-half-edge's twin among the in-half-edges and back
+  __device__ __forceinline__ void pair() {
 #pragma unroll
     for (int i = 0; i < CAP; ++i) {
       jm_[i] = -1;
@@ -373,7 +424,7 @@ __device__ __forceinline__ void row_symbolic(const Row& r, int32_t v, int32_t& c
 template <int SCHEME, class Row>
 __device__ __forceinline__ void row_number(Row& r, const LevelView& L, int32_t v) {
   constexpr int C = Row::kCap;
-  const int k = seg_of(L.seg.vbase, L.seg.n, v);
+  const int k = seg_of_wave(L.seg.vbase, L.seg.n, v);
   const SegDesc* __restrict__ S = L.seg.d + k;
   const int64_t ne_cap = S->ne;
   const int64_t eb = S->eb;

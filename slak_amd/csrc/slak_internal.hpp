@@ -55,7 +55,9 @@ void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int3
 void launch_transpose(bool quad, const int32_t* ids, const int32_t* fptr, int32_t nf, const int32_t* tptr,
                       int32_t* cnt, int32_t* tids, const int* guard, hipStream_t st);
 void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st);
-// first level: edge numbering and (faces set) face points in one launch
+// first level: edge numbering and (faces set) face points in one launch; the
+// rows come from the symbolic pass (L.trow sorted with in-neighbours, L.tids
+// sorted in place for rows above the register capacity)
 void launch_number_faces(bool quad, int scheme, int cap, const LevelView& L, bool faces, hipStream_t st);
 void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st);
 // refine (per corner) and vertex (per vertex) items of a level in one launch

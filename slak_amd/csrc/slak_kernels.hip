@@ -149,7 +149,7 @@ struct BlockHist {
 
 __device__ void gather_face(const SegTable& seg, int32_t f, int32_t nf_total, int use_vals, int32_t* __restrict__ ids_pos,
                             int32_t* __restrict__ fptr, int32_t* __restrict__ cface, int* __restrict__ flags,
-                            BlockHist* hist);
+                            int32_t* hist);
 
 __global__ void __launch_bounds__(kBlock) k_gather(SegTable seg, int32_t nf_total, int use_vals, int32_t* __restrict__ ids_pos,
                          int32_t* __restrict__ fptr, int32_t* __restrict__ cface, int* __restrict__ flags,
@@ -159,16 +159,16 @@ __global__ void __launch_bounds__(kBlock) k_gather(SegTable seg, int32_t nf_tota
   if (hist_faces) bh.init();
   const int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (f < nzero) zero[f] = 0;  // counters of the analysis that follows
-  if (f < nf_total) gather_face(seg, f, nf_total, use_vals, ids_pos, fptr, cface, flags, hist_faces ? &bh : nullptr);
+  if (f < nf_total) gather_face(seg, f, nf_total, use_vals, ids_pos, fptr, cface, flags, hist_faces ? h : nullptr);
   if (hist_faces) bh.flush(hist_faces);
 }
 
 // first level: validate caller face f and gather it into the union in
-// position order; face order into the block histogram
+// position order; face order into the block's LDS histogram (hist, or null)
 __device__ void gather_face(const SegTable& seg, int32_t f, int32_t nf_total, int use_vals, int32_t* __restrict__ ids_pos,
                             int32_t* __restrict__ fptr, int32_t* __restrict__ cface, int* __restrict__ flags,
-                            BlockHist* hist) {
-  const int k = seg_of(seg.fbase, seg.n, f);
+                            int32_t* hist) {
+  const int k = seg_of_wave(seg.fbase, seg.n, f);
   const SegDesc* S = seg.d + k;
   const int32_t lf = (int32_t)(f - S->fb);
   const int32_t* sp = S->src_ptr;
@@ -187,11 +187,58 @@ __device__ void gather_face(const SegTable& seg, int32_t f, int32_t nf_total, in
     return;
   }
   const int32_t order = (int32_t)(e - s);
-  if (hist) hist->add(order);
+  if (hist) atomicAdd(&hist[order < kHistBins - 1 ? order : kHistBins - 1], 1);
   if (order < 3) bad |= kFlagBadOrder;
   if (S->require_quad && (order != 4 || s != 4ll * lf)) bad |= kFlagBadQuad;
   const int64_t cb = S->cb;
   const int32_t vb = (int32_t)S->vb;
+  if (order == 4 && ((reinterpret_cast<uintptr_t>(sid + s) | (sval ? reinterpret_cast<uintptr_t>(sval + s) : 0)) & 15) == 0) {
+    // a quad with 16-byte aligned corners: one load of its ids (and vals),
+    // one store when the union position is aligned too; anything unusual
+    // takes the per-corner path below
+    const int4 q = *reinterpret_cast<const int4*>(sid + s);
+    const int4 p = sval ? *reinterpret_cast<const int4*>(sval + s) : make_int4(1, 2, 3, 4);
+    const uint32_t in_range = (uint32_t)q.x < (uint32_t)S->nv && (uint32_t)q.y < (uint32_t)S->nv &&
+                              (uint32_t)q.z < (uint32_t)S->nv && (uint32_t)q.w < (uint32_t)S->nv;
+    const bool pos_ok = (uint32_t)(p.x - 1) < 4u && (uint32_t)(p.y - 1) < 4u && (uint32_t)(p.z - 1) < 4u &&
+                        (uint32_t)(p.w - 1) < 4u;
+    if (in_range && pos_ok && ((1u << (p.x - 1)) | (1u << (p.y - 1)) | (1u << (p.z - 1)) | (1u << (p.w - 1))) == 15u) {
+      // the corner at position k (1-based) is the one whose vals entry is k
+      auto at = [&](int k) __attribute__((always_inline)) {
+        return vb + (p.x == k ? q.x : p.y == k ? q.y : p.z == k ? q.z : q.w);
+      };
+      const int4 o = make_int4(at(1), at(2), at(3), at(4));
+      int32_t* dst = ids_pos + cb + s;
+      if ((reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
+        *reinterpret_cast<int4*>(dst) = o;
+      } else {
+        dst[0] = o.x;
+        dst[1] = o.y;
+        dst[2] = o.z;
+        dst[3] = o.w;
+      }
+      if (cface) {
+        int32_t* cf = cface + cb + s;
+        if ((reinterpret_cast<uintptr_t>(cf) & 15) == 0) {
+          *reinterpret_cast<int4*>(cf) = make_int4(f, f, f, f);
+        } else {
+          cf[0] = f;
+          cf[1] = f;
+          cf[2] = f;
+          cf[3] = f;
+        }
+      }
+      if (fptr) {
+        fptr[f] = (int32_t)(cb + s);
+        if (f == nf_total - 1) fptr[nf_total] = (int32_t)(cb + e);
+      }
+      if (bad) {
+        atomicOr(seg.mflags + k, bad);
+        atomicOr(flags, bad);
+      }
+      return;
+    }
+  }
   uint32_t mask = 0;
   for (int64_t c = s; c < e; ++c) {
     int32_t id = sid[c];
@@ -228,7 +275,7 @@ __device__ void gather_face(const SegTable& seg, int32_t f, int32_t nf_total, in
 __global__ void k_gather_verts(SegTable seg, int32_t nv_total, float4* __restrict__ P) {
   int32_t v = (int32_t)(block_index() * blockDim.x + threadIdx.x);
   if (v >= nv_total) return;
-  const int k = seg_of(seg.vbase, seg.n, v);
+  const int k = seg_of_wave(seg.vbase, seg.n, v);
   const SegDesc* S = seg.d + k;
   P[v] = S->src_verts[v - S->vb];
 }
@@ -252,7 +299,7 @@ __device__ __forceinline__ void faces_item(const int32_t* __restrict__ ids, cons
   if (guard && (*guard & kFlagsInput)) return;
   float4* dst = nullptr;
   if (FP) {
-    const SegDesc* S = seg.d + seg_of(seg.fbase, seg.n, f);
+    const SegDesc* S = seg.d + seg_of_wave(seg.fbase, seg.n, f);
     dst = S->dst_verts + S->nv + (f - S->fb);
   }
   if (QUAD) {
@@ -322,37 +369,70 @@ __device__ void sort_row_in_place(int32_t* a, int n) {
   }
 }
 
+// The M^T rows of a block's 256 consecutive vertices are one contiguous
+// segment [tptr[v0], tptr[v0 + 256]) of tids (and of trow): when it fits the
+// block's LDS tile it is read in with consecutive lanes on consecutive
+// entries, the rows are taken from LDS, and the sorted rows with their
+// in-neighbours go back out the same way. Longer segments use global memory.
+constexpr int kRowTile = 1536;  // entries (rows averaging 6): 6 KB of corners in, 12 KB of (corner, in-neighbour) out
+
 template <bool QUAD, int SCHEME, int CAP>
-__device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, BlockHist* hist);
+__device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, int32_t* hist, const int32_t* rows_in,
+                                             int2* rows_out, int32_t s0);
 
 template <bool QUAD, int SCHEME, int CAP>
 __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
   __shared__ int32_t h[kHistBins];
+  __shared__ int32_t tile_in[kRowTile];
+  __shared__ int2 tile_out[kRowTile];
   BlockHist bh{h};
   if (L.hist_rows) bh.init();
-  const int32_t v = (int32_t)(block_index() * blockDim.x + threadIdx.x);
-  if (v < L.nv && !(*L.flags & kFlagsInput)) symbolic_row<QUAD, SCHEME, CAP>(L, v, L.hist_rows ? &bh : nullptr);
+  const int64_t v0 = block_index() * blockDim.x;
+  const int32_t v = (int32_t)(v0 + threadIdx.x);
+  const bool valid = !(*L.flags & kFlagsInput);
+  int32_t s0 = 0, len = 0;
+  if (valid && v0 < L.nv) {
+    s0 = L.tptr[v0];
+    len = L.tptr[min(v0 + (int64_t)blockDim.x, (int64_t)L.nv)] - s0;
+  }
+  const bool staged = len > 0 && len <= kRowTile;  // block-uniform
+  if (staged)
+    for (int i = threadIdx.x; i < len; i += blockDim.x) tile_in[i] = L.tids[s0 + i];
+  __syncthreads();
+  if (v < L.nv && valid)
+    symbolic_row<QUAD, SCHEME, CAP>(L, v, L.hist_rows ? h : nullptr, staged ? tile_in : nullptr,
+                                    staged && L.trow ? tile_out : nullptr, s0);
+  __syncthreads();
+  if (staged && L.trow)
+    for (int i = threadIdx.x; i < len; i += blockDim.x) L.trow[s0 + i] = tile_out[i];
   if (L.hist_rows) bh.flush(L.hist_rows);
 }
 
 // mapped-SpGEMM row sizing of vertex v (edges it owns), validation, the
-// sorted row with in-neighbours; row length into the block histogram
+// sorted row with in-neighbours; row length into the block histogram.
+// rows_in / rows_out: the block's LDS tiles from position s0, or null
 template <bool QUAD, int SCHEME, int CAP>
-__device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, BlockHist* hist) {
+__device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, int32_t* hist, const int32_t* rows_in,
+                                             int2* rows_out, int32_t s0) {
   int32_t s = L.tptr[v];
   int n = L.tptr[v + 1] - s;
-  if (hist) hist->add(n);
+  if (hist) atomicAdd(&hist[n < kHistBins - 1 ? n : kHistBins - 1], 1);
   int32_t count = 0, bcount = 0;
   int bad = 0;
-  int2* __restrict__ trow = L.trow;
+  // the sorted row goes to the block's LDS tile (or straight to trow)
+  auto put = [&](int i, int2 x) __attribute__((always_inline)) {
+    if (rows_out)
+      rows_out[s - s0 + i] = x;
+    else if (L.trow)
+      L.trow[s + i] = x;
+  };
   if (n <= CAP) {
     RegRow<CAP, QUAD> r;
-    r.load(L, s, n);
+    r.load(L, rows_in ? rows_in + (s - s0) : L.tids + s, n);
     row_symbolic<SCHEME>(r, v, count, bcount, bad);
-    if (trow)
 #pragma unroll
-      for (int i = 0; i < CAP; ++i)
-        if (i < n) trow[s + i] = make_int2(r.T(i), r.U(i));
+    for (int i = 0; i < CAP; ++i)
+      if (i < n) put(i, make_int2(r.T(i), r.U(i)));
     if (L.mpos)
 #pragma unroll
       for (int i = 0; i < CAP; ++i)
@@ -361,8 +441,7 @@ __device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, Bloc
     sort_row_in_place(L.tids + s, n);
     MemRow<QUAD> r(L, s, n);
     row_symbolic<SCHEME>(r, v, count, bcount, bad);
-    if (trow)
-      for (int i = 0; i < n; ++i) trow[s + i] = make_int2(r.T(i), r.U(i));
+    for (int i = 0; i < n; ++i) put(i, make_int2(r.T(i), r.U(i)));
     if (L.mpos)
       for (int i = 0; i < n; ++i) L.mpos[r.T(i)] = s + i;
   }
@@ -375,7 +454,7 @@ __device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, Bloc
 }
 
 template <bool QUAD, int SCHEME, int CAP>
-__device__ __forceinline__ void number_row(const LevelView& L, int32_t v);
+__device__ __forceinline__ void number_row(const LevelView& L, int32_t v, const int2* rows, int32_t s0);
 template <bool QUAD, bool FP, bool COUNT>
 __device__ __forceinline__ void faces_item(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr,
                                            const float4* __restrict__ P, int32_t nf, const SegTable& seg,
@@ -386,21 +465,34 @@ __device__ __forceinline__ void faces_item(const int32_t* __restrict__ ids, cons
 template <bool QUAD, int SCHEME, int CAP>
 __global__ void __launch_bounds__(kBlock) k_number_faces(LevelView L, int32_t nbn) {
   const int64_t b = block_index();
-  if (b < nbn)
-    number_row<QUAD, SCHEME, CAP>(L, (int32_t)(b * blockDim.x + threadIdx.x));
-  else
+  if (b < nbn) {
+    // the block's sorted rows through LDS, as in k_symbolic
+    __shared__ int2 tile[kRowTile];
+    const int64_t v0 = b * blockDim.x;
+    int32_t s0 = 0, len = 0;
+    if (v0 < L.nv) {
+      s0 = L.tptr[v0];
+      len = L.tptr[min(v0 + (int64_t)blockDim.x, (int64_t)L.nv)] - s0;
+    }
+    const bool staged = len > 0 && len <= kRowTile;  // block-uniform
+    if (staged)
+      for (int i = threadIdx.x; i < len; i += blockDim.x) tile[i] = L.trow[s0 + i];
+    __syncthreads();
+    number_row<QUAD, SCHEME, CAP>(L, (int32_t)(v0 + threadIdx.x), staged ? tile : nullptr, s0);
+  } else
     faces_item<QUAD, true, false>(L.ids, L.fptr, L.P, L.nf, L.seg, nullptr, nullptr,
                                   (int32_t)((b - nbn) * blockDim.x + threadIdx.x));
 }
 
+// rows: the block's LDS copy of trow from position s0, or null
 template <bool QUAD, int SCHEME, int CAP>
-__device__ __forceinline__ void number_row(const LevelView& L, int32_t v) {
+__device__ __forceinline__ void number_row(const LevelView& L, int32_t v, const int2* rows, int32_t s0) {
   if (v >= L.nv) return;
   int32_t s = L.tptr[v];
   int n = L.tptr[v + 1] - s;
   if (n <= CAP) {
     RegRow<CAP, QUAD> r;
-    r.load(L, s, n);
+    r.load_sorted(L, rows ? rows + (s - s0) : L.trow + s, n);  // the symbolic pass's sorted rows
     row_number<SCHEME>(r, L, v);
   } else {
     // the symbolic pass sorted every row longer than its own capacity; the
@@ -452,7 +544,7 @@ struct SegK {
 
 __device__ __forceinline__ SegK seg_consts(const SegTable& seg, int32_t f) {
   SegK K;
-  K.S = seg.d + seg_of(seg.fbase, seg.n, f);
+  K.S = seg.d + seg_of_wave(seg.fbase, seg.n, f);
   K.vb = (int32_t)K.S->vb;
   K.fb = (int32_t)K.S->fb;
   K.cb = (int32_t)K.S->cb;
@@ -699,7 +791,7 @@ constexpr int kVertexRegs = 4;
 template <bool QUAD, int SCHEME, bool NEXT, bool BND>
 __device__ __forceinline__ void vertex_item(const RefineView& R, int32_t v) {
   if (v >= R.nv) return;
-  const SegDesc* __restrict__ S = R.seg.d + seg_of(R.seg.vbase, R.seg.n, v);
+  const SegDesc* __restrict__ S = R.seg.d + seg_of_wave(R.seg.vbase, R.seg.n, v);
   const int32_t vb = (int32_t)S->vb, fb = (int32_t)S->fb, cb = (int32_t)S->cb;
   const int32_t snv = S->nv;
   float4* __restrict__ dst = S->dst_verts;
@@ -886,11 +978,14 @@ __device__ __forceinline__ float4 face_point(float4 a, float4 b, float4 c, float
 
 constexpr int kStageStride = 5;  // int4 per lane in the staging area (4 used; 5 spreads the banks)
 
-// Corner t (mesh-local) of M in mesh S; its 16 output ids go to the block's
-// staging area.
+struct Quads4 {
+  int4 a, b, c, d;
+};
+
+// Corner t (mesh-local) of M in mesh S: returns its 16 output ids (IDS),
+// which the block stages in LDS.
 template <int SCHEME, bool BND, bool IDS>
-__device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg& S, int32_t t,
-                                             int4* __restrict__ stage) {
+__device__ __forceinline__ Quads4 final_corner(const FinalView& V, const FinalSeg& S, int32_t t) {
   // every lane of a quad runs (nnz = 4 nf, meshes and blocks hold whole quads)
   const int64_t cb = S.cb;
   const int32_t vb = (int32_t)S.vb;
@@ -991,16 +1086,16 @@ __device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg&
     ew = s0 < 0 ? 2 * nnz + bl0 : mp_s0;
   }
   // the four output quads of M1 face t
+  Quads4 q4{};
   if (IDS) {
     const int32_t off = S.id_off, eb2 = epb2 + off;
     const int32_t x0 = v + off, x1 = ep0 + e1 + off, x2 = nv + F + off, x3 = ep0 + e0 + off, fid = nv1 + t + off;
-    // into the block's staging area: the block writes them out as 1 KB runs
+    // the block stages them in LDS and writes them out as 1 KB runs
     // (64-byte pieces per lane, stored directly, cost several times more)
-    int4* __restrict__ o = stage + kStageStride * (threadIdx.x & (kBlock - 1));
-    o[0] = make_int4(x0, eb2 + ex, fid, eb2 + ew);
-    o[1] = make_int4(x1, eb2 + ey, fid, eb2 + ex);
-    o[2] = make_int4(x2, eb2 + ez, fid, eb2 + ey);
-    o[3] = make_int4(x3, eb2 + ew, fid, eb2 + ez);
+    q4.a = make_int4(x0, eb2 + ex, fid, eb2 + ew);
+    q4.b = make_int4(x1, eb2 + ey, fid, eb2 + ex);
+    q4.c = make_int4(x2, eb2 + ez, fid, eb2 + ey);
+    q4.d = make_int4(x3, eb2 + ew, fid, eb2 + ez);
   }
   if (SCHEME == kSLAK) {
     // the M1 edge from ep(e1) to fp(F), SLAK slot nnz + next(t): every lane
@@ -1013,7 +1108,7 @@ __device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg&
     else
       *d = q;
   }
-  if (!own) return;
+  if (!own) return q4;
   // edge points of the M1 edges around ep(e1), by lower end point then the
   // faces of the half-edges lower -> upper and upper -> lower
   const bool inner = !BND || s1 >= 0;
@@ -1070,6 +1165,7 @@ __device__ __forceinline__ void final_corner(const FinalView& V, const FinalSeg&
     q = f4add(f4add(f4mul(Pe, 0.75f), f4mul(Pw, 0.125f)), f4mul(Pv, 0.125f));
   }
   out[ep0 + e1] = q;
+  return q4;
 }
 
 // vertex point of a vertex v of M in the output: row 4t for its corners t in
@@ -1202,12 +1298,20 @@ __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int3
     }
     const int32_t t = t0 + (int32_t)threadIdx.x;
     if (!IDS) {  // positions only
-      if ((int)threadIdx.x < cnt) final_corner<SCHEME, BND, false>(V, S, t, nullptr);
+      if ((int)threadIdx.x < cnt) final_corner<SCHEME, BND, false>(V, S, t);
       return;
     }
     __shared__ int4 stage[kStageStride * kBlock];
-    if ((int)threadIdx.x < cnt) final_corner<SCHEME, BND, true>(V, S, t, stage);
-    __syncthreads();  // the whole block: corner blocks are uniform
+    Quads4 q4{};
+    if ((int)threadIdx.x < cnt) q4 = final_corner<SCHEME, BND, true>(V, S, t);
+    if ((int)threadIdx.x < cnt) {
+      int4* __restrict__ o = stage + kStageStride * threadIdx.x;
+      o[0] = q4.a;
+      o[1] = q4.b;
+      o[2] = q4.c;
+      o[3] = q4.d;
+    }
+    __syncthreads();
     const int nq = 4 * cnt;
     int32_t* __restrict__ dst = S.out_ids + 16 * (int64_t)t0;
 #pragma unroll
@@ -1245,7 +1349,7 @@ __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int3
 __global__ void __launch_bounds__(kBlock) k_lak_form(SegTable seg, int64_t nnz) {
   const int64_t t = block_index() * blockDim.x + threadIdx.x;
   if (t >= nnz) return;
-  const SegDesc* __restrict__ S = seg.d + seg_of(seg.cbase, seg.n, t);
+  const SegDesc* __restrict__ S = seg.d + seg_of_wave(seg.cbase, seg.n, t);
   if (!S->dst_ptr) return;
   const int32_t lt = (int32_t)(t - S->cb);
   store_nt(S->dst_ptr + lt, 4 * lt);
@@ -1430,14 +1534,28 @@ __global__ void __launch_bounds__(kScanSmallThreads) k_scan_small(const int32_t*
   }
 }
 
+// Tiles of kScanTile items: item (r, tid, j) of a tile is r * 4 kBlock + 4 tid
+// + j (r < kScanRounds, j < 4), so that every load and store instruction of a
+// wave covers 1 KB of consecutive ints.
+constexpr int kScanRounds = kScanItems / 4;
+static_assert(kScanRounds * 4 * kBlock == kScanTile, "tile layout");
+
+__device__ __forceinline__ int4 scan_load4(const int32_t* __restrict__ in, int64_t i, int64_t n, bool vec) {
+  if (vec && i + 3 < n) return *reinterpret_cast<const int4*>(in + i);
+  return make_int4(i < n ? in[i] : 0, i + 1 < n ? in[i + 1] : 0, i + 2 < n ? in[i + 2] : 0, i + 3 < n ? in[i + 3] : 0);
+}
+
 __global__ void __launch_bounds__(kBlock) k_scan_reduce(const int32_t* __restrict__ in, int64_t n,
                                                         int64_t* __restrict__ partial) {
   __shared__ int64_t lds[kBlock / 64];
-  int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  const bool vec = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + 4 * threadIdx.x;
+  int4 x[kScanRounds];
+#pragma unroll
+  for (int r = 0; r < kScanRounds; ++r) x[r] = scan_load4(in, base + (int64_t)r * 4 * kBlock, n, vec);
   int64_t s = 0;
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k)
-    if (base + k < n) s += in[base + k];
+  for (int r = 0; r < kScanRounds; ++r) s += (int64_t)x[r].x + x[r].y + x[r].z + x[r].w;
   int64_t tot;
   block_exclusive_scan(s, lds, tot);
   if (threadIdx.x == 0) partial[blockIdx.x] = tot;
@@ -1465,21 +1583,48 @@ __global__ void __launch_bounds__(kBlock) k_scan_partials(int64_t* __restrict__ 
 
 __global__ void __launch_bounds__(kBlock) k_scan_down(const int32_t* __restrict__ in, int64_t n,
                                                       const int64_t* __restrict__ partial, int32_t* __restrict__ out) {
-  __shared__ int64_t lds[kBlock / 64];
-  int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-  int32_t x[kScanItems];
-  int64_t s = 0;
+  constexpr int kWaves = kBlock / 64;
+  __shared__ int64_t wsum[kScanRounds * kWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool vin = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+  const bool vout = (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + 4 * threadIdx.x;
+  int4 x[kScanRounds];
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    x[k] = (base + k < n) ? in[base + k] : 0;
-    s += x[k];
+  for (int r = 0; r < kScanRounds; ++r) x[r] = scan_load4(in, base + (int64_t)r * 4 * kBlock, n, vin);
+  int64_t g[kScanRounds], v[kScanRounds];
+#pragma unroll
+  for (int r = 0; r < kScanRounds; ++r) {
+    g[r] = (int64_t)x[r].x + x[r].y + x[r].z + x[r].w;
+    v[r] = wave_scan64(g[r]);
   }
-  int64_t tot;
-  int64_t run = block_exclusive_scan(s, lds, tot) + partial[blockIdx.x];
+  if (lane == 63)
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    if (base + k < n) out[base + k] = (int32_t)run;
-    run += x[k];
+    for (int r = 0; r < kScanRounds; ++r) wsum[r * kWaves + wave] = v[r];
+  __syncthreads();
+  if (threadIdx.x == 0) {  // the wave totals, rounds major: exclusive prefixes in place
+    int64_t run = partial[blockIdx.x];
+    for (int k = 0; k < kScanRounds * kWaves; ++k) {
+      const int64_t t = wsum[k];
+      wsum[k] = run;
+      run += t;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kScanRounds; ++r) {
+    const int64_t e0 = wsum[r * kWaves + wave] + v[r] - g[r];
+    const int4 o = make_int4((int32_t)e0, (int32_t)(e0 + x[r].x), (int32_t)(e0 + x[r].x + x[r].y),
+                             (int32_t)(e0 + x[r].x + x[r].y + x[r].z));
+    const int64_t i = base + (int64_t)r * 4 * kBlock;
+    if (vout && i + 3 < n) {
+      *reinterpret_cast<int4*>(out + i) = o;
+    } else {
+      if (i < n) out[i] = o.x;
+      if (i + 1 < n) out[i + 1] = o.y;
+      if (i + 2 < n) out[i + 2] = o.z;
+      if (i + 3 < n) out[i + 3] = o.w;
+    }
   }
 }
 
@@ -1591,6 +1736,7 @@ void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStre
 }
 
 void launch_number_faces(bool quad, int scheme, int cap, const LevelView& L, bool faces, hipStream_t st) {
+  if (!L.trow) throw Error(5, "edge numbering without the symbolic pass's rows");  // SLAK_ERR_INTERNAL
   const int32_t nbn = L.nv > 0 ? (int32_t)grid_for(L.nv) : 0;
   const int32_t nbf = faces && L.nf > 0 ? (int32_t)grid_for(L.nf) : 0;
   if (nbn + nbf == 0) return;

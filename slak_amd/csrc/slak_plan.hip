@@ -866,6 +866,7 @@ void Plan::launch_number(LevelPlan& L, bool faces) {
   V.nnz = L.nnz;
   V.tptr = L.tptr;
   V.tids = L.tids;
+  V.trow = L.trow;  // sorted rows with in-neighbours (symbolic pass)
   V.eptr = L.eptr;
   V.ci = reinterpret_cast<int32_t*>(L.ci);
   V.bnd = L.bnd;
