@@ -148,31 +148,33 @@ def fetch_result(m):
     return ids, verts, ptr, vals
 
 
-def batched_leg(spec, scheme, steps, warmup, keep_results=64):
-    """cfg4: the plain form (one subdivideIteratively per mesh) and the
-    batched chain (graph replays). Returns the headline numbers, the profile
-    of one eager run and the first keep_results meshes' outputs (the parity
-    sample)."""
+def batched_leg(spec, scheme, steps, warmup, keep_results=64, plain=True):
+    """cfg4: the plain form (one subdivideIteratively per mesh; skipped with
+    plain=False) and the batched chain (graph replays). Returns the headline
+    numbers, the profile of one eager run and the first keep_results meshes'
+    outputs (the parity sample)."""
     from slak_amd import capi
     from slak_amd.batch import Batch
     from slak_amd.mesh import MeshInfo, _subdivide
 
     dev = [MeshInfo.from_host(h) for h in spec.meshes]
     try:
-        for d, lv in list(zip(dev, spec.levels))[:64]:  # warm the per-mesh path
-            r = MeshInfo()
-            _subdivide(scheme, d, r, lv)
-            r.freeAndReset()
-        capi.check(capi.lib().slak_device_synchronize())
-        t0 = time.perf_counter()
-        faces = 0
-        for d, lv in zip(dev, spec.levels):
-            r = MeshInfo()
-            st = _subdivide(scheme, d, r, lv)
-            faces += int(st.out_faces)
-            r.freeAndReset()
-        capi.check(capi.lib().slak_device_synchronize())
-        plain_s = time.perf_counter() - t0
+        plain_s, faces = None, None
+        if plain:
+            for d, lv in list(zip(dev, spec.levels))[:64]:  # warm the per-mesh path
+                r = MeshInfo()
+                _subdivide(scheme, d, r, lv)
+                r.freeAndReset()
+            capi.check(capi.lib().slak_device_synchronize())
+            t0 = time.perf_counter()
+            faces = 0
+            for d, lv in zip(dev, spec.levels):
+                r = MeshInfo()
+                st = _subdivide(scheme, d, r, lv)
+                faces += int(st.out_faces)
+                r.freeAndReset()
+            capi.check(capi.lib().slak_device_synchronize())
+            plain_s = time.perf_counter() - t0
 
         t0 = time.perf_counter()
         b = Batch(dev, spec.levels, scheme)
@@ -190,6 +192,8 @@ def batched_leg(spec, scheme, steps, warmup, keep_results=64):
             capi.check(capi.lib().slak_device_synchronize())
             secs = time.perf_counter() - t0
             inf = b.info()
+            if faces is None:
+                faces = int(inf.out_faces)
             assert int(inf.out_faces) == faces, "batched and per-mesh face counts differ"
             capi.profile_enable(True)
             b.run(use_graph=False)
@@ -567,6 +571,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle parity check")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of single-thread CPU baseline work")
     ap.add_argument("--no-legs", action="store_true", help="skip the cfg2 / cfg3 single-mesh legs")
+    ap.add_argument("--no-plain", action="store_true", help="skip the per-mesh plain form of the batch")
+    ap.add_argument("--leg", choices=["cfg2", "cfg3"], help="run only this single-mesh leg (profiling runs)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-qps-fraction", type=float, default=0.5)
@@ -582,6 +588,16 @@ def main():
               "with a one-ring halo). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.leg:  # one single-mesh chain, for kernel traces
+        from slak_amd import capi
+        scheme = capi.SLAK_SCHEME_LAK if args.scheme == "lak" else capi.SLAK_SCHEME_SLAK
+        m = make_single(args.leg)
+        lv = SINGLE[args.leg][1]
+        secs, faces, st, prof = time_single(m, lv, scheme, args.steps, args.warmup)
+        print(json.dumps({"leg": args.leg, "scheme": args.scheme, "faces_per_s": round(faces * args.steps / secs, 1),
+                          "ms_per_chain": round(secs / args.steps * 1e3, 4), "stages": stage_table(prof)}), flush=True)
+        return
+
     # HBM counters first, in child processes, before this one touches the GPU
     traffic = None if args.no_traffic else hbm_traffic(args.scheme, args.batch_meshes)
 
@@ -596,7 +612,7 @@ def main():
 
     # headline: the 4096-mesh batch, graph replays
     spec = meshgen.config4(n_meshes=args.batch_meshes)
-    B = batched_leg(spec, scheme, args.steps, args.warmup)
+    B = batched_leg(spec, scheme, args.steps, args.warmup, plain=not args.no_plain)
     secs = B["secs"]
     value = B["faces"] * args.steps / secs
     roof = add_traffic(stage_roofline(B["prof"], CFG4_STAGE), traffic, "cfg4", args.scheme, True)
@@ -626,7 +642,7 @@ def main():
         "roofline": roof,
         "io_floor": io_floor(floor_b, secs / args.steps),
         "copy_kernel_gbps": round(copy_gbps.value, 1),
-        "batched_vs_plain": {
+        "batched_vs_plain": None if B["plain_s"] is None else {
             "plain_per_mesh_s": round(B["plain_s"], 4),
             "batched_graph_s": round(secs / args.steps, 5),
             "speedup_vs_plain": round(B["plain_s"] / (secs / args.steps), 2),

@@ -122,7 +122,7 @@ struct LevelView {
   int2* trow;            // sorted rows with in-neighbours (written by the symbolic pass), or null
   int32_t* mpos;         // M^T storage position of every corner (SLAK, symbolic pass), or null
   int32_t* ecnt;
-  int32_t* hist_rows;    // zeroed 17-bin histogram of row lengths (symbolic pass), or null
+  int32_t* hist_rows;    // zeroed histogram copies of row lengths (symbolic pass), or null
   const int32_t* eptr;
   int32_t* ci;           // (edge id, twin) pairs per corner, written by the number pass
   int32_t* bnd;          // union edge boundary flags (number pass), or null: closed level
@@ -418,9 +418,9 @@ __device__ __forceinline__ void row_symbolic(const Row& r, int32_t v, int32_t& c
 
 // Number pass of the first level: the mesh-local id of every edge v owns
 // (LAK: column v of E, spgemm.cuh:71-100; SLAK: flagged out-half-edges in
-// corner order, CatmullClarkKernels.cuh:147-236) stored at both of its
-// half-edges' corners, the twin corner of each out-half-edge of v, and the
-// boundary flag of every owned edge by union edge index.
+// corner order, CatmullClarkKernels.cuh:147-236) and the twin corner, stored
+// as (edge id, twin) at both of its half-edges' corners, and the boundary
+// flag of every owned edge by union edge index.
 template <int SCHEME, class Row>
 __device__ __forceinline__ void row_number(Row& r, const LevelView& L, int32_t v) {
   constexpr int C = Row::kCap;
@@ -442,9 +442,12 @@ __device__ __forceinline__ void row_number(Row& r, const LevelView& L, int32_t v
     }
     return e;
   };
-  auto put = [ci, bnd, eb](int32_t t, int32_t tw, int32_t e, bool boundary) __attribute__((always_inline)) {
-    ci[2 * (int64_t)t] = e;
-    if (tw >= 0) ci[2 * (int64_t)tw] = e;
+  // The owner of an edge writes both of its half-edges' (edge id, twin)
+  // pairs whole: every corner's pair is written once, by one 8-byte store.
+  int2* __restrict__ ci2 = reinterpret_cast<int2*>(ci);
+  auto put = [ci2, bnd, eb](int32_t t, int32_t tw, int32_t e, bool boundary) __attribute__((always_inline)) {
+    ci2[t] = make_int2(e, tw);
+    if (tw >= 0) ci2[tw] = make_int2(e, t);
     if (bnd) bnd[eb + e] = boundary ? 1 : 0;
   };
   if (SCHEME == kLAK) {
@@ -467,9 +470,6 @@ __device__ __forceinline__ void row_number(Row& r, const LevelView& L, int32_t v
       }
     });
   }
-  rloop<C>(n, [&](int i) __attribute__((always_inline)) {
-    ci[2 * (int64_t)r.T(i) + 1] = r.JM(i) >= 0 ? r.TPJ(i) : -1;
-  });
 }
 
 // One level's refinement inputs and the next level's structure.

@@ -39,11 +39,25 @@ struct Error : std::runtime_error {
 void init_device_options();
 void launch_fill_i32(int32_t* p, int64_t n, int32_t value, hipStream_t st);
 // also zeroes zero[0..nzero) (the counters of the analysis that follows) and,
-// when hist_faces is set, adds the face orders to that zeroed 17-bin histogram
+// when hist_faces is set, adds the face orders to that zeroed histogram block
+// (kHistCopies copies of 17 bins; the host sums the copies)
 void launch_gather(const SegTable& seg, int32_t nf_total, bool use_vals, int32_t* ids_pos, int32_t* fptr,
                    int32_t* cface, int* flags, int32_t* zero, int32_t nzero, int32_t* hist_faces, hipStream_t st);
 void launch_gather_verts(const SegTable& seg, int32_t nv_total, float4* P, hipStream_t st);
 constexpr int kRowHistBins = 17;  // row lengths 0..15, 16 = longer
+// Each histogram is kept as kHistCopies copies, one 128-byte line apart; block
+// b adds its counts to copy b % kHistCopies. A single copy takes one
+// device-scope atomic per block on the same address, and those serialize: on
+// a 4096-block launch they cost about 35 us.
+constexpr int kHistCopies = 32;
+constexpr int kHistStride = 32;  // ints per copy
+constexpr int kHistInts = kHistCopies * kHistStride;
+// The plan's state block (int32, zeroed before the gather), read back from
+// its start after the first level's analysis:
+// [flags | pad | row hist copies | face hist copies | edge bounds (nseg + 1) | mesh flags (nseg)]
+constexpr int kStateRowHist = 32;
+constexpr int kStateFaceHist = kStateRowHist + kHistInts;
+constexpr int kStateBounds = kStateFaceHist + kHistInts;
 // bounds[k] = eptr[vbase[k]], k = 0..nseg
 void launch_seg_bounds(const int64_t* vbase, int32_t nseg, const int32_t* eptr, int32_t* bounds, hipStream_t st);
 void launch_copy_probe(const float4* a, float4* b, int64_t n, hipStream_t st);

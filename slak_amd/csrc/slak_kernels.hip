@@ -130,20 +130,36 @@ __global__ void __launch_bounds__(kBlock) k_digest(const uint8_t* __restrict__ p
 // first level: validate each caller mesh and gather it into the union in
 // position order (sortKeyValuePairsSegmentedInPlace, LAKCatmullClarkKernels.cuh:147-180)
 
-constexpr int kHistBins = 17;  // row lengths 0..15, 16 = longer
+constexpr int kHistBins = kRowHistBins;  // row lengths 0..15, 16 = longer
+static_assert(kHistBins <= kHistStride, "a histogram copy holds every bin");
 
-// Block-local histogram in LDS, added to a zeroed global one (one atomic per bin).
+// Adds 1 to bin `bin` of an LDS histogram for every calling lane, with one
+// LDS atomic per distinct bin of the wave: same-address LDS atomics from the
+// lanes of a wave serialize (on a quad mesh every lane has the same bin).
+__device__ __forceinline__ void hist_add_wave(int32_t* lds, int bin) {
+  unsigned long long todo = __ballot(1);
+  while (todo) {
+    const int leader = __ffsll(todo) - 1;
+    const int b = __shfl(bin, leader);
+    const unsigned long long same = __ballot(bin == b) & todo;
+    if ((int)(threadIdx.x & 63) == leader) atomicAdd(&lds[b], __popcll(same));
+    todo &= ~same;
+  }
+}
+
+// Block-local histogram in LDS, added to copy blockIdx % kHistCopies of a
+// zeroed global histogram block (one atomic per nonzero bin).
 struct BlockHist {
   int32_t* lds;
   __device__ __forceinline__ void init() {
     for (int i = threadIdx.x; i < kHistBins; i += blockDim.x) lds[i] = 0;
     __syncthreads();
   }
-  __device__ __forceinline__ void add(int n) { atomicAdd(&lds[n < kHistBins - 1 ? n : kHistBins - 1], 1); }
   __device__ __forceinline__ void flush(int32_t* global) {
     __syncthreads();
+    int32_t* copy = global + (blockIdx.x % kHistCopies) * kHistStride;
     for (int i = threadIdx.x; i < kHistBins; i += blockDim.x)
-      if (lds[i]) atomicAdd(&global[i], lds[i]);
+      if (lds[i]) atomicAdd(&copy[i], lds[i]);
   }
 };
 
@@ -187,7 +203,7 @@ __device__ void gather_face(const SegTable& seg, int32_t f, int32_t nf_total, in
     return;
   }
   const int32_t order = (int32_t)(e - s);
-  if (hist) atomicAdd(&hist[order < kHistBins - 1 ? order : kHistBins - 1], 1);
+  if (hist) hist_add_wave(hist, order < kHistBins - 1 ? order : kHistBins - 1);
   if (order < 3) bad |= kFlagBadOrder;
   if (S->require_quad && (order != 4 || s != 4ll * lf)) bad |= kFlagBadQuad;
   const int64_t cb = S->cb;
@@ -416,7 +432,7 @@ __device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, int3
                                              int2* rows_out, int32_t s0) {
   int32_t s = L.tptr[v];
   int n = L.tptr[v + 1] - s;
-  if (hist) atomicAdd(&hist[n < kHistBins - 1 ? n : kHistBins - 1], 1);
+  if (hist) hist_add_wave(hist, n < kHistBins - 1 ? n : kHistBins - 1);
   int32_t count = 0, bcount = 0;
   int bad = 0;
   // the sorted row goes to the block's LDS tile (or straight to trow)
@@ -990,7 +1006,7 @@ __device__ __forceinline__ Quads4 final_corner(const FinalView& V, const FinalSe
   const int64_t cb = S.cb;
   const int32_t vb = (int32_t)S.vb;
   const int32_t F = t >> 2, q0 = t & ~3;
-  const int32_t tn = q0 | ((t + 1) & 3), tp = q0 | ((t + 3) & 3);
+  const int32_t tn = q0 | ((t + 1) & 3);
   const int32_t* __restrict__ ids = V.ids + cb;
   const int2* __restrict__ ci = V.ci + cb;
   // the quad's four lanes load their own corner once; w (next corner's

@@ -216,7 +216,7 @@ void Plan::upload_tables(int l0, int l1, Staging& staging, int64_t state_ints) {
     L.d_cbase = reinterpret_cast<int64_t*>(d + o) + 2 * (n + 1);
     if (!L.d_mflags) {
       if (l == 0 && d_state_)  // the first level's mesh flags live in the state block
-        L.d_mflags = d_state_ + 1 + 2 * kRowHistBins + (n + 1);
+        L.d_mflags = d_state_ + kStateBounds + (n + 1);
       else
         L.d_mflags = arena_.get_n<int>((int64_t)std::max<size_t>(n, 1));
     }
@@ -364,7 +364,7 @@ void Plan::record_analysis(LevelPlan& L) {
   V.trow = L.trow;
   V.mpos = L.mpos;
   V.ecnt = L.ecnt;
-  V.hist_rows = d_state_ + 1;
+  V.hist_rows = d_state_ + kStateRowHist;
   V.flags = d_flags_;
   V.seg = L.table();
   {
@@ -374,7 +374,7 @@ void Plan::record_analysis(LevelPlan& L) {
   }
   {
     // a single mesh's edge total goes straight to its read-back slot
-    int32_t* bounds = d_state_ + 1 + 2 * kRowHistBins;
+    int32_t* bounds = d_state_ + kStateBounds;
     ProfScope ps(st_, "scan", 8.0 * L.nv + 4);
     launch_scan(L.ecnt, L.eptr, L.nv, d_partial_, d_flags_, st_, L.segs.size() == 1 ? bounds + 1 : nullptr);
   }
@@ -427,7 +427,7 @@ void Plan::prepare() {
       L1.segs[0].dst_verts = early_p1_;
     }
   }
-  n_state_ = 1 + 2 * kRowHistBins + (nseg + 1) + nseg;
+  n_state_ = kStateBounds + (nseg + 1) + nseg;
   upload_tables(0, 1, staging_first_, n_state_);
   int64_t need = scan_partials_len(L1.nv);
   d_partial_ = arena_.get_n<int64_t>(need);
@@ -438,7 +438,7 @@ void Plan::prepare() {
     ProfScope ps(st_, "gather", 4.0 * (L1.nf + 1) + 12.0 * L1.nnz + (L1.quad ? 0.0 : 8.0 * L1.nnz + 4.0 * L1.nf));
     launch_gather(L1.table(), (int32_t)L1.nf, scheme_ == kLAK, const_cast<int32_t*>(L1.ids),
                   const_cast<int32_t*>(L1.fptr), const_cast<int32_t*>(L1.cface), d_flags_, d_cnt_, (int32_t)L1.nv,
-                  d_state_ + 1 + kRowHistBins, st_);
+                  d_state_ + kStateFaceHist, st_);
   }
   if (P1_) launch_gather_verts(L1.table(), (int32_t)L1.nv, P1_, st_);
   auto fail_if = [&](int fl) {
@@ -457,14 +457,22 @@ void Plan::prepare() {
   // (its kernels do nothing on an input the gather flagged; the flags come
   // back with the edge counts in one round trip)
   record_analysis(L1);
-  int32_t* d_bounds = d_state_ + 1 + 2 * kRowHistBins;
+  int32_t* d_bounds = d_state_ + kStateBounds;
   if (nseg > 1) launch_seg_bounds(L1.d_vbase, nseg, L1.eptr, d_bounds, st_);
   std::vector<int64_t> hb(2 * kRowHistBins + nseg + 1, 0);
   bool have_hist = false;
   if (!all_known) {
     std::function<void()> overlap;
     if (early) overlap = [this, &L1] { launch_number(L1, early_p1_ != nullptr); };
-    fail_if(readback(nullptr, 2 * kRowHistBins + nseg + 1, hb.data(), overlap));
+    // state ints 1 .. kStateBounds + nseg: flags, the histogram copies, the edge bounds
+    std::vector<int64_t> sb((size_t)kStateBounds + nseg, 0);
+    fail_if(readback(nullptr, kStateBounds + nseg, sb.data(), overlap));
+    for (int c = 0; c < kHistCopies; ++c)
+      for (int b = 0; b < kRowHistBins; ++b) {
+        hb[b] += sb[kStateRowHist - 1 + c * kHistStride + b];
+        hb[kRowHistBins + b] += sb[kStateFaceHist - 1 + c * kHistStride + b];
+      }
+    for (int k = 0; k <= nseg; ++k) hb[2 * kRowHistBins + k] = sb[kStateBounds - 1 + k];
     // the early numbering was recorded before its edge count was known: its
     // boundary-flag writes (4 bytes per edge) join its algorithmic bytes now
     if (early && Profiler::get().enabled())
@@ -1128,7 +1136,7 @@ void Plan::launch(bool include_analysis) {
       ProfScope ps(st_, "gather", 4.0 * (L1.nf + 1) + 12.0 * L1.nnz + (L1.quad ? 0.0 : 8.0 * L1.nnz + 4.0 * L1.nf));
       launch_gather(L1.table(), (int32_t)L1.nf, scheme_ == kLAK, const_cast<int32_t*>(L1.ids),
                     const_cast<int32_t*>(L1.fptr), const_cast<int32_t*>(L1.cface), d_flags_, d_cnt_,
-                    (int32_t)L1.nv, d_state_ + 1 + kRowHistBins, st_);
+                    (int32_t)L1.nv, d_state_ + kStateFaceHist, st_);
     }
     if (P1_) {
       ProfScope ps(st_, "gather_verts", 32.0 * L1.nv);
