@@ -1129,7 +1129,7 @@ void Plan::launch(bool include_analysis) {
   if (levels_.empty()) return;
   if (include_analysis) number_early_ = false;  // the analysis runs again: number in order
   LevelPlan& L1 = levels_[0];
-  if (!lak_forms_in_place_) launch_lak_forms();
+  launch_lak_forms();  // on the second stream; a graph captures the fork and join
   if (include_analysis) {
     launch_fill_i32(d_state_, n_state_, 0, st_);  // a fill kernel: graph-safe
     {
@@ -1179,13 +1179,6 @@ void Plan::run_graph() {
     // event-pair timing stays out of the captured graph
     const bool prof = Profiler::get().enabled();
     Profiler::get().set_enabled(false);
-    // the LAK forms are constant: written once here, outside the graph
-    launch_lak_forms();
-    if (lak_forms_pending_) {
-      SLAK_HIP_CHECK(hipStreamWaitEvent(st_, aux_stream().join, 0));
-      lak_forms_pending_ = false;
-    }
-    lak_forms_in_place_ = true;
     SLAK_HIP_CHECK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
     try {
       launch(true);

@@ -183,7 +183,6 @@ class Plan {
   hipStream_t owned_st_ = nullptr;    // created when the caller's stream is the default one
   hipEvent_t order_ev_ = nullptr;
   bool lak_forms_pending_ = false;
-  bool lak_forms_in_place_ = false;  // written before a graph capture; replays keep them
   // some mesh refined two or more levels with a quad mesh two levels before
   // its output: its last level is computed from that mesh's structure
   // (k_final) and the level before it writes positions only

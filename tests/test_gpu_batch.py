@@ -203,3 +203,37 @@ def test_batch_reports_the_invalid_mesh():
     finally:
         for d in dev:
             d.freeAndReset()
+
+
+@pytest.mark.parametrize("scheme", [LAK, SLAK], ids=["LAK", "SLAK"])
+def test_graph_replay_rewrites_every_output(scheme):
+    """Every output of the batch (ids, positions and the LAK form's ptr and
+    vals, whichever path forms a mesh's last level) is overwritten by each
+    graph replay: poison them all between replays and the next replay must
+    restore the first replay's bits."""
+    hosts, levels = mixed_batch(40, seed=21)
+    hosts += [g.torus(6, 5, seed=2, mesh=90), g.torus(7, 4, seed=3, mesh=91)]
+    levels += [3, 4]
+    dev = [MeshInfo.from_host(h) for h in hosts]
+    try:
+        with Batch(dev, levels, scheme) as b:
+            b.run(use_graph=True)
+            live = [i for i in range(len(hosts)) if b.result(i).ids != 0]
+            first = {i: fetch(b.result(i)) for i in live}
+            for rep in range(2):
+                for i in live:
+                    r = b.result(i)
+                    for ptr, n in ((r.ids, r.nnz), (r.verts, 4 * r.nverts), (r.ptr, r.nfaces + 1), (r.vals, r.nnz)):
+                        if ptr:
+                            junk = np.full(n, -7, dtype=np.int32)
+                            capi.check(capi.lib().slak_memcpy_h2d(ptr, junk.ctypes.data, junk.nbytes))
+                b.run(use_graph=True)
+                for i in live:
+                    for got, want in zip(fetch(b.result(i)), first[i]):
+                        if want is None:
+                            assert got is None
+                        else:
+                            np.testing.assert_array_equal(got, want, err_msg=f"mesh {i}, replay {rep}")
+    finally:
+        for d in dev:
+            d.freeAndReset()
