@@ -136,6 +136,8 @@ struct LevelView {
   int32_t* D_ids;
   int32_t* D_vals;
   int64_t ne_cap;  // E capacity of the SpGEMM operator
+  int32_t rows_per_block;  // first-level row kernels: M^T rows per block (64, 128 or 256)
+  int32_t tile_entries;    // first-level row kernels: entries of the block's LDS row tile
 };
 
 __device__ __forceinline__ float4 f4add(float4 a, float4 b) {

@@ -45,6 +45,10 @@ void launch_gather(const SegTable& seg, int32_t nf_total, bool use_vals, int32_t
                    int32_t* cface, int* flags, int32_t* zero, int32_t nzero, int32_t* hist_faces, hipStream_t st);
 void launch_gather_verts(const SegTable& seg, int32_t nv_total, float4* P, hipStream_t st);
 constexpr int kRowHistBins = 17;  // row lengths 0..15, 16 = longer
+// LDS tile of the first level's row kernels, in M^T entries (sized per level
+// by the plan, at most this): a block's rows are staged through it when their
+// total length fits
+constexpr int kRowTileMax = 2560;
 // Each histogram is kept as kHistCopies copies, one 128-byte line apart; block
 // b adds its counts to copy b % kHistCopies. A single copy takes one
 // device-scope atomic per block on the same address, and those serialize: on

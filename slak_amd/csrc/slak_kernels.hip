@@ -385,12 +385,14 @@ __device__ void sort_row_in_place(int32_t* a, int n) {
   }
 }
 
-// The M^T rows of a block's 256 consecutive vertices are one contiguous
-// segment [tptr[v0], tptr[v0 + 256]) of tids (and of trow): when it fits the
-// block's LDS tile it is read in with consecutive lanes on consecutive
-// entries, the rows are taken from LDS, and the sorted rows with their
-// in-neighbours go back out the same way. Longer segments use global memory.
-constexpr int kRowTile = 1536;  // entries (rows averaging 6): 6 KB of corners in, 12 KB of (corner, in-neighbour) out
+// The M^T rows of a block's R consecutive vertices (R = L.rows_per_block:
+// 256, 128 or 64) are one contiguous segment [tptr[v0], tptr[v0 + R]) of tids
+// (and of trow): when it fits the block's LDS tile (L.tile_entries entries,
+// dynamic LDS; the plan sizes R and the tile from the row lengths) it is read
+// in with all 256 lanes on consecutive entries, the rows are taken from LDS
+// (one lane per row), and the sorted rows with their in-neighbours go back out
+// the same way. Longer segments use global memory.
+extern __shared__ __align__(16) unsigned char row_tile_lds[];
 
 template <bool QUAD, int SCHEME, int CAP>
 __device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, int32_t* hist, const int32_t* rows_in,
@@ -399,23 +401,25 @@ __device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, int3
 template <bool QUAD, int SCHEME, int CAP>
 __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
   __shared__ int32_t h[kHistBins];
-  __shared__ int32_t tile_in[kRowTile];
-  __shared__ int2 tile_out[kRowTile];
+  const int T = L.tile_entries;  // even: tile_out starts 8-byte aligned
+  int32_t* const tile_in = reinterpret_cast<int32_t*>(row_tile_lds);       // T corners
+  int2* const tile_out = reinterpret_cast<int2*>(row_tile_lds + 4 * T);  // T (corner, in-neighbour)
   BlockHist bh{h};
   if (L.hist_rows) bh.init();
-  const int64_t v0 = block_index() * blockDim.x;
+  const int R = L.rows_per_block;
+  const int64_t v0 = block_index() * R;
   const int32_t v = (int32_t)(v0 + threadIdx.x);
   const bool valid = !(*L.flags & kFlagsInput);
   int32_t s0 = 0, len = 0;
   if (valid && v0 < L.nv) {
     s0 = L.tptr[v0];
-    len = L.tptr[min(v0 + (int64_t)blockDim.x, (int64_t)L.nv)] - s0;
+    len = L.tptr[min(v0 + (int64_t)R, (int64_t)L.nv)] - s0;
   }
-  const bool staged = len > 0 && len <= kRowTile;  // block-uniform
+  const bool staged = len > 0 && len <= T;  // block-uniform
   if (staged)
     for (int i = threadIdx.x; i < len; i += blockDim.x) tile_in[i] = L.tids[s0 + i];
   __syncthreads();
-  if (v < L.nv && valid)
+  if ((int)threadIdx.x < R && v < L.nv && valid)
     symbolic_row<QUAD, SCHEME, CAP>(L, v, L.hist_rows ? h : nullptr, staged ? tile_in : nullptr,
                                     staged && L.trow ? tile_out : nullptr, s0);
   __syncthreads();
@@ -483,18 +487,19 @@ __global__ void __launch_bounds__(kBlock) k_number_faces(LevelView L, int32_t nb
   const int64_t b = block_index();
   if (b < nbn) {
     // the block's sorted rows through LDS, as in k_symbolic
-    __shared__ int2 tile[kRowTile];
-    const int64_t v0 = b * blockDim.x;
+    int2* const tile = reinterpret_cast<int2*>(row_tile_lds);  // L.tile_entries entries
+    const int R = L.rows_per_block;
+    const int64_t v0 = b * R;
     int32_t s0 = 0, len = 0;
     if (v0 < L.nv) {
       s0 = L.tptr[v0];
-      len = L.tptr[min(v0 + (int64_t)blockDim.x, (int64_t)L.nv)] - s0;
+      len = L.tptr[min(v0 + (int64_t)R, (int64_t)L.nv)] - s0;
     }
-    const bool staged = len > 0 && len <= kRowTile;  // block-uniform
+    const bool staged = len > 0 && len <= L.tile_entries;  // block-uniform
     if (staged)
       for (int i = threadIdx.x; i < len; i += blockDim.x) tile[i] = L.trow[s0 + i];
     __syncthreads();
-    number_row<QUAD, SCHEME, CAP>(L, (int32_t)(v0 + threadIdx.x), staged ? tile : nullptr, s0);
+    if ((int)threadIdx.x < R) number_row<QUAD, SCHEME, CAP>(L, (int32_t)(v0 + threadIdx.x), staged ? tile : nullptr, s0);
   } else
     faces_item<QUAD, true, false>(L.ids, L.fptr, L.P, L.nf, L.seg, nullptr, nullptr,
                                   (int32_t)((b - nbn) * blockDim.x + threadIdx.x));
@@ -1724,54 +1729,67 @@ void launch_transpose(bool quad, const int32_t* ids, const int32_t* fptr, int32_
 
 // cap: M^T row length held in registers (4 or 8); longer rows take the
 // in-memory path. Chosen per level from the row-length histogram.
-#define SLAK_ROW_DISPATCH(KERNEL, quad, scheme, cap, g, L, st)                       \
-  do {                                                                               \
-    if (quad) {                                                                      \
-      if (scheme == kLAK) {                                                          \
-        if (cap <= 4) KERNEL<true, kLAK, 4><<<g, kBlock, 0, st>>>(L);                \
-        else KERNEL<true, kLAK, 8><<<g, kBlock, 0, st>>>(L);                         \
-      } else {                                                                       \
-        if (cap <= 4) KERNEL<true, kSLAK, 4><<<g, kBlock, 0, st>>>(L);               \
-        else KERNEL<true, kSLAK, 8><<<g, kBlock, 0, st>>>(L);                        \
-      }                                                                              \
-    } else {                                                                         \
-      if (scheme == kLAK) {                                                          \
-        if (cap <= 4) KERNEL<false, kLAK, 4><<<g, kBlock, 0, st>>>(L);               \
-        else KERNEL<false, kLAK, 8><<<g, kBlock, 0, st>>>(L);                        \
-      } else {                                                                       \
-        if (cap <= 4) KERNEL<false, kSLAK, 4><<<g, kBlock, 0, st>>>(L);              \
-        else KERNEL<false, kSLAK, 8><<<g, kBlock, 0, st>>>(L);                       \
-      }                                                                              \
-    }                                                                                \
+#define SLAK_ROW_DISPATCH(KERNEL, quad, scheme, cap, g, smem, L, st)                  \
+  do {                                                                                \
+    if (quad) {                                                                       \
+      if (scheme == kLAK) {                                                           \
+        if (cap <= 4) KERNEL<true, kLAK, 4><<<g, kBlock, smem, st>>>(L);              \
+        else KERNEL<true, kLAK, 8><<<g, kBlock, smem, st>>>(L);                       \
+      } else {                                                                        \
+        if (cap <= 4) KERNEL<true, kSLAK, 4><<<g, kBlock, smem, st>>>(L);             \
+        else KERNEL<true, kSLAK, 8><<<g, kBlock, smem, st>>>(L);                      \
+      }                                                                               \
+    } else {                                                                          \
+      if (scheme == kLAK) {                                                           \
+        if (cap <= 4) KERNEL<false, kLAK, 4><<<g, kBlock, smem, st>>>(L);             \
+        else KERNEL<false, kLAK, 8><<<g, kBlock, smem, st>>>(L);                      \
+      } else {                                                                        \
+        if (cap <= 4) KERNEL<false, kSLAK, 4><<<g, kBlock, smem, st>>>(L);            \
+        else KERNEL<false, kSLAK, 8><<<g, kBlock, smem, st>>>(L);                     \
+      }                                                                               \
+    }                                                                                 \
   } while (0)
+
+// rows per block (64, 128 or 256) and LDS tile (even, 256..kRowTileMax
+// entries) of the first level's row kernels
+static void check_row_tiling(const LevelView& L) {
+  if (L.rows_per_block != 64 && L.rows_per_block != 128 && L.rows_per_block != kBlock)
+    throw Error(5, "rows per block must be 64, 128 or 256");  // SLAK_ERR_INTERNAL
+  if (L.tile_entries < 256 || L.tile_entries > kRowTileMax || (L.tile_entries & 1))
+    throw Error(5, "row tile out of range");  // SLAK_ERR_INTERNAL
+}
 
 void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStream_t st) {
   if (L.nv <= 0) return;
-  unsigned g = grid_for(L.nv);
-  SLAK_ROW_DISPATCH(k_symbolic, quad, scheme, cap, g, L, st);
+  check_row_tiling(L);
+  unsigned g = grid_for(L.nv, L.rows_per_block);
+  const size_t smem = (size_t)L.tile_entries * (sizeof(int32_t) + sizeof(int2));
+  SLAK_ROW_DISPATCH(k_symbolic, quad, scheme, cap, g, smem, L, st);
 }
 
 void launch_number_faces(bool quad, int scheme, int cap, const LevelView& L, bool faces, hipStream_t st) {
   if (!L.trow) throw Error(5, "edge numbering without the symbolic pass's rows");  // SLAK_ERR_INTERNAL
-  const int32_t nbn = L.nv > 0 ? (int32_t)grid_for(L.nv) : 0;
+  check_row_tiling(L);
+  const int32_t nbn = L.nv > 0 ? (int32_t)grid_for(L.nv, L.rows_per_block) : 0;
+  const size_t smem = (size_t)L.tile_entries * sizeof(int2);
   const int32_t nbf = faces && L.nf > 0 ? (int32_t)grid_for(L.nf) : 0;
   if (nbn + nbf == 0) return;
   const unsigned g = (unsigned)(nbn + nbf);
   if (quad) {
     if (scheme == kLAK) {
-      if (cap <= 4) k_number_faces<true, kLAK, 4><<<g, kBlock, 0, st>>>(L, nbn);
-      else k_number_faces<true, kLAK, 8><<<g, kBlock, 0, st>>>(L, nbn);
+      if (cap <= 4) k_number_faces<true, kLAK, 4><<<g, kBlock, smem, st>>>(L, nbn);
+      else k_number_faces<true, kLAK, 8><<<g, kBlock, smem, st>>>(L, nbn);
     } else {
-      if (cap <= 4) k_number_faces<true, kSLAK, 4><<<g, kBlock, 0, st>>>(L, nbn);
-      else k_number_faces<true, kSLAK, 8><<<g, kBlock, 0, st>>>(L, nbn);
+      if (cap <= 4) k_number_faces<true, kSLAK, 4><<<g, kBlock, smem, st>>>(L, nbn);
+      else k_number_faces<true, kSLAK, 8><<<g, kBlock, smem, st>>>(L, nbn);
     }
   } else {
     if (scheme == kLAK) {
-      if (cap <= 4) k_number_faces<false, kLAK, 4><<<g, kBlock, 0, st>>>(L, nbn);
-      else k_number_faces<false, kLAK, 8><<<g, kBlock, 0, st>>>(L, nbn);
+      if (cap <= 4) k_number_faces<false, kLAK, 4><<<g, kBlock, smem, st>>>(L, nbn);
+      else k_number_faces<false, kLAK, 8><<<g, kBlock, smem, st>>>(L, nbn);
     } else {
-      if (cap <= 4) k_number_faces<false, kSLAK, 4><<<g, kBlock, 0, st>>>(L, nbn);
-      else k_number_faces<false, kSLAK, 8><<<g, kBlock, 0, st>>>(L, nbn);
+      if (cap <= 4) k_number_faces<false, kSLAK, 4><<<g, kBlock, smem, st>>>(L, nbn);
+      else k_number_faces<false, kSLAK, 8><<<g, kBlock, smem, st>>>(L, nbn);
     }
   }
 }

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 
@@ -103,6 +104,53 @@ int autotune_cap(bool quad, int64_t nv, int64_t long_rows, int max_row) {
   double frac = (double)long_rows / (double)nv;
   (void)quad;
   return frac < thresh ? 4 : 8;
+}
+
+// Rows per block and LDS row tile of the first level's row kernels
+// (k_symbolic, the numbering of k_number_faces). A block of R rows needs
+// about R m + 3 sqrt(R) sigma tile entries, with m the mean row length
+// (corners over vertices) and sigma the spread of row lengths from the
+// histogram (hist: kRowHistBins counts, the last bin "longer"; without one,
+// sigma = 1): R is the largest of 256, 128, 64 whose need fits kRowTileMax,
+// and the tile that need rounded up to 256 entries (at least 1024). A block
+// whose rows exceed the tile reads them from global memory. On levels of
+// 2^18 vertices or more the numbering takes half as many rows per block (at
+// least 64). Medians of interleaved runs on a 1024 x 1024 torus: numbering
+// 38.6 -> 35.0 us with quads, 89.5 -> 72.5 us with triangles, where the sizing
+// pass is fastest at R = 256 (33.3 against 43.5 us, 62.8 against 74.7); on
+// the 256 x 128 mixed torus (32768 vertices) the numbering is faster at 256
+// (11.8 against 13.5 us). SLAK_ROWS_PER_BLOCK forces the sizing pass's R,
+// SLAK_ROWS_NUMBER the numbering's.
+RowTiling autotune_rows(int64_t nv, int64_t nnz, const int64_t* hist) {
+  double m = nv > 0 ? (double)nnz / (double)nv : 4.0;
+  double sigma = 1.0;
+  if (hist) {
+    double n = 0, sq = 0;
+    for (int b = 0; b < kRowHistBins; ++b) {
+      n += (double)hist[b];
+      sq += (double)hist[b] * (b - m) * (b - m);
+    }
+    if (n > 0) sigma = std::sqrt(sq / n);
+  }
+  auto need = [&](int R) { return R * m + 3.0 * std::sqrt((double)R) * sigma; };
+  int R = 64;
+  for (int r : {256, 128})
+    if (need(r) <= (double)kRowTileMax) {
+      R = r;
+      break;
+    }
+  if (const char* e = std::getenv("SLAK_ROWS_PER_BLOCK")) {
+    const int forced = std::atoi(e);
+    if (forced == 64 || forced == 128 || forced == 256) R = forced;
+  }
+  const double want = std::ceil(need(R) / 256.0) * 256.0;
+  const int tile = (int)std::min<double>(kRowTileMax, std::max(1024.0, want));
+  int Rn = nv >= (int64_t(1) << 18) ? std::max(64, R / 2) : R;
+  if (const char* e = std::getenv("SLAK_ROWS_NUMBER")) {
+    const int forced = std::atoi(e);
+    if (forced == 64 || forced == 128 || forced == 256) Rn = forced;
+  }
+  return RowTiling{R, Rn, tile};
 }
 
 Plan::Plan(int scheme, std::vector<MeshJob> jobs, OutputMode mode, hipStream_t st)
@@ -296,6 +344,10 @@ void Plan::plan_first_level() {
   check_i32(L.nf, "faces of the first level");
   check_i32(L.nnz, "corners of the first level");
   check_i32(4 * L.nnz, "refined corners of the first level");
+  const RowTiling rt = autotune_rows(L.nv, L.nnz, nullptr);
+  L.rows = rt.rows;
+  L.rows_num = rt.rows_num;
+  L.tile = rt.tile;
 
   // one arena block for the first level's buffers and the analysis state
   // that follows (the table upload, the scan partials, a single mesh's
@@ -365,6 +417,8 @@ void Plan::record_analysis(LevelPlan& L) {
   V.mpos = L.mpos;
   V.ecnt = L.ecnt;
   V.hist_rows = d_state_ + kStateRowHist;
+  V.rows_per_block = L.rows;
+  V.tile_entries = L.tile;
   V.flags = d_flags_;
   V.seg = L.table();
   {
@@ -821,6 +875,14 @@ void Plan::prepare() {
     L.max_row = l == 1 ? vmax : std::max(std::max(vmax, fmax), 4);
     L.cap = autotune_cap(L.quad, L.nv, L.long_rows, L.max_row);  // used by the first level's row kernels
   }
+  // the first level's rows per block, now with the row-length spread (the
+  // numbering after the read-back and every graph replay use it)
+  {
+    const RowTiling rt = autotune_rows(levels_[0].nv, levels_[0].nnz, have_hist ? hb.data() : nullptr);
+    levels_[0].rows = rt.rows;
+    levels_[0].rows_num = rt.rows_num;
+    levels_[0].tile = rt.tile;
+  }
   // k_final tables: per fused mesh, its segment of M (the level before's
   // union), the positions of M1 and its outputs
   std::vector<int> seg_in_m(jobs_.size(), -1);  // job -> its segment of M
@@ -880,6 +942,8 @@ void Plan::launch_number(LevelPlan& L, bool faces) {
   V.bnd = L.bnd;
   V.flags = d_flags_;
   V.seg = L.table();
+  V.rows_per_block = L.rows_num;
+  V.tile_entries = L.tile;
   ProfScope ps(st_, faces ? (L.quad ? "number_facepoints_quad" : "number_facepoints_poly") : "number",
                12.0 * L.nv + 12.0 * L.nnz + 8.0 * L.nnz + (L.bnd ? 4.0 * ne : 0.0) + poly_b +
                    (faces ? 4.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf : 0.0));

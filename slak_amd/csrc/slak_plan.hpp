@@ -56,6 +56,13 @@ struct FinalTable {
   bool empty() const { return segs.empty(); }
 };
 
+// rows per block and LDS row tile (entries) of the first level's row kernels
+struct RowTiling {
+  int rows;      // k_symbolic
+  int rows_num;  // the numbering of k_number_faces
+  int tile;
+};
+
 struct LevelPlan {
   bool quad = true;
   int64_t nv = 0, nf = 0, nnz = 0;  // union sizes
@@ -66,6 +73,9 @@ struct LevelPlan {
   int64_t ne = 0;           // union edges
   bool boundary = false;    // some mesh of the level has boundary edges
   int cap = 8;              // first level: M^T row length held in registers (autotuned)
+  int rows = 256;           // first level: M^T rows per block of k_symbolic (autotuned)
+  int rows_num = 128;       // first level: M^T rows per block of the numbering (autotuned)
+  int tile = 1536;          // first level: LDS row tile of the row kernels, entries (autotuned)
   int64_t long_rows = -1;   // rows longer than 4 (from the first level's histogram)
   int max_row = -1;
   // first-level analysis scratch

@@ -211,6 +211,46 @@ def test_high_valence_vs_oracle(oracle, scheme, name, make, cap, monkeypatch):
         check_oracle(oracle, m, levels, scheme, (out.ids, out.verts, out.ptr, out.vals), f"{name} L{levels}")
 
 
+ROW_MESHES = [
+    ("tri_torus", lambda: g.torus_mixed(40, 30, 1.0, seed=4)),  # valence 6 everywhere
+    ("uv_sphere24", lambda: g.uv_sphere(24, 6)),
+    ("mix", lambda: g.high_valence_mix()),
+    ("mixed_torus", lambda: g.torus_mixed(48, 32, 0.25, pentagons=True, seed=5)),
+]
+
+
+@pytest.mark.parametrize("scheme", [LAK, SLAK], ids=["LAK", "SLAK"])
+def test_rows_per_block_choices_agree(oracle, scheme, monkeypatch):
+    """The first level's row kernels take 64, 128 or 256 rows per block (the
+    plan picks one from the row lengths): every choice gives the same bits,
+    alone and in a batch replayed as a graph, and the oracle's topology."""
+    hosts = [make() for _, make in ROW_MESHES]
+    base = [subdivide_host(h, 2, scheme)[0] for h in hosts]
+    for i, h in enumerate(hosts):
+        check_oracle(oracle, h, 2, scheme, (base[i].ids, base[i].verts, base[i].ptr, base[i].vals), ROW_MESHES[i][0])
+    for rows in ("64", "128", "256"):
+        monkeypatch.setenv("SLAK_ROWS_PER_BLOCK", rows)
+        for i, h in enumerate(hosts):
+            out, _ = subdivide_host(h, 2, scheme)
+            for got, want in zip((out.ids, out.verts, out.ptr, out.vals), (base[i].ids, base[i].verts, base[i].ptr, base[i].vals)):
+                if want is None:
+                    assert got is None
+                else:
+                    np.testing.assert_array_equal(got, want, err_msg=f"{ROW_MESHES[i][0]} rows {rows}")
+        dev = [MeshInfo.from_host(h) for h in hosts]
+        try:
+            with Batch(dev, [2] * len(hosts), scheme) as b:
+                for _ in range(2):
+                    b.run(use_graph=True)
+                    for i in range(len(hosts)):
+                        ids, verts, ptr, vals = fetch(b.result(i))
+                        np.testing.assert_array_equal(ids, base[i].ids, err_msg=f"batch {i} rows {rows}")
+                        np.testing.assert_array_equal(verts, base[i].verts, err_msg=f"batch {i} rows {rows}")
+        finally:
+            for d in dev:
+                d.freeAndReset()
+
+
 @pytest.mark.parametrize("cap", ["4", "8"])
 def test_high_valence_spgemm_operator(oracle, cap, monkeypatch):
     """F, D and E of the mapped SpGEMM operator on rows of up to 24 corners."""
