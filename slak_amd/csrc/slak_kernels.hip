@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
 #include <mutex>
 #include <vector>
@@ -346,15 +347,152 @@ __global__ void k_faces(const int32_t* __restrict__ ids, const int32_t* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Valence count and transpose through a block-local vertex window. The corners
+// of a block's faces usually reference a narrow range of vertex ids (grids,
+// refined meshes, each mesh of a batch): the block then counts into LDS over
+// [vmin, vmax] and adds each touched vertex's count with one device atomic,
+// and the transpose reserves each touched vertex's slots with one atomic and
+// hands them out by LDS atomics. Device-scope atomics are what both passes
+// cost (one per corner otherwise); a block whose window is wider than
+// kCountWindow vertices takes one per corner.
+constexpr int kCountWindow = 2048;
+
+// min / max over the block (mm: kBlock / 64 LDS entries)
+__device__ __forceinline__ int2 block_minmax(int lo, int hi, int2* mm) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    lo = min(lo, __shfl_xor(lo, d));
+    hi = max(hi, __shfl_xor(hi, d));
+  }
+  if ((threadIdx.x & 63) == 0) mm[threadIdx.x >> 6] = make_int2(lo, hi);
+  __syncthreads();
+  int2 r = mm[0];
+#pragma unroll
+  for (int w = 1; w < kBlock / 64; ++w) r = make_int2(min(r.x, mm[w].x), max(r.y, mm[w].y));
+  return r;
+}
+
+// the vertex range of face f's corners (quad: c holds them)
+template <bool QUAD>
+__device__ __forceinline__ int2 face_range(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr,
+                                           int32_t f, int4& c, int32_t& s, int32_t& e) {
+  if (QUAD) {
+    c = reinterpret_cast<const int4*>(ids)[f];
+    return make_int2(min(min(c.x, c.y), min(c.z, c.w)), max(max(c.x, c.y), max(c.z, c.w)));
+  }
+  s = fptr[f];
+  e = fptr[f + 1];
+  int lo = INT_MAX, hi = INT_MIN;
+  for (int32_t k = s; k < e; ++k) {
+    const int32_t v = ids[k];
+    lo = min(lo, v);
+    hi = max(hi, v);
+  }
+  return make_int2(lo, hi);
+}
+
+// cnt[v] += valence of v over the faces [0, nf) (cnt zeroed before)
+template <bool QUAD>
+__global__ void __launch_bounds__(kBlock) k_count(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr,
+                                                  int32_t nf, int32_t* __restrict__ cnt, const int* __restrict__ guard) {
+  __shared__ int32_t win[kCountWindow];
+  __shared__ int2 mm[kBlock / 64];
+  if (guard && (*guard & kFlagsInput)) return;  // block-uniform
+  const int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
+  const bool act = f < nf;
+  int4 c = make_int4(0, 0, 0, 0);
+  int32_t s = 0, e = 0;
+  int2 r = make_int2(INT_MAX, INT_MIN);
+  if (act) r = face_range<QUAD>(ids, fptr, f, c, s, e);
+  const int2 w = block_minmax(r.x, r.y, mm);
+  const int64_t n = (int64_t)w.y - w.x + 1;  // <= 0 when no lane is active
+  if (n <= kCountWindow) {                    // block-uniform
+    for (int i = threadIdx.x; i < n; i += blockDim.x) win[i] = 0;
+    __syncthreads();
+    if (act) {
+      if (QUAD) {
+        atomicAdd(&win[c.x - w.x], 1);
+        atomicAdd(&win[c.y - w.x], 1);
+        atomicAdd(&win[c.z - w.x], 1);
+        atomicAdd(&win[c.w - w.x], 1);
+      } else {
+        for (int32_t k = s; k < e; ++k) atomicAdd(&win[ids[k] - w.x], 1);
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int32_t k = win[i];
+      if (k) atomicAdd(&cnt[w.x + i], k);
+    }
+  } else if (act) {
+    if (QUAD) {
+      atomicAdd(&cnt[c.x], 1);
+      atomicAdd(&cnt[c.y], 1);
+      atomicAdd(&cnt[c.z], 1);
+      atomicAdd(&cnt[c.w], 1);
+    } else {
+      for (int32_t k = s; k < e; ++k) atomicAdd(&cnt[ids[k]], 1);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // transpose scatter: tids[tptr[v] + slot] = corner; cnt[] counts back to zero
+// (slot order within a row is arbitrary: the row-sizing pass sorts the rows)
 
 template <bool QUAD>
-__global__ void k_transpose(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr, int32_t nf,
-                            const int32_t* __restrict__ tptr, int32_t* __restrict__ cnt, int32_t* __restrict__ tids,
-                            const int* __restrict__ guard) {
-  int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
-  if (f >= nf) return;
-  if (guard && (*guard & kFlagsInput)) return;
+__global__ void __launch_bounds__(kBlock) k_transpose(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr,
+                                                      int32_t nf, const int32_t* __restrict__ tptr,
+                                                      int32_t* __restrict__ cnt, int32_t* __restrict__ tids,
+                                                      const int* __restrict__ guard, int window) {
+  __shared__ int32_t lc[kCountWindow];    // the block's corners per window vertex, then counted back
+  __shared__ int32_t base[kCountWindow];  // the first tids slot the block reserved for each
+  __shared__ int2 mm[kBlock / 64];
+  if (guard && (*guard & kFlagsInput)) return;  // block-uniform
+  const int32_t f = (int32_t)(block_index() * blockDim.x + threadIdx.x);
+  const bool act = f < nf;
+  int4 c = make_int4(0, 0, 0, 0);
+  int32_t s = 0, e = 0;
+  int2 r = make_int2(INT_MAX, INT_MIN);
+  if (act) r = face_range<QUAD>(ids, fptr, f, c, s, e);
+  const int2 w = block_minmax(r.x, r.y, mm);
+  const int64_t n = (int64_t)w.y - w.x + 1;
+  if (window && n <= kCountWindow) {  // block-uniform
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lc[i] = 0;
+    __syncthreads();
+    if (act) {
+      if (QUAD) {
+        atomicAdd(&lc[c.x - w.x], 1);
+        atomicAdd(&lc[c.y - w.x], 1);
+        atomicAdd(&lc[c.z - w.x], 1);
+        atomicAdd(&lc[c.w - w.x], 1);
+      } else {
+        for (int32_t k = s; k < e; ++k) atomicAdd(&lc[ids[k] - w.x], 1);
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      const int32_t k = lc[i];
+      if (k) base[i] = tptr[w.x + i] + atomicSub(&cnt[w.x + i], k) - k;
+    }
+    __syncthreads();
+    if (act) {
+      if (QUAD) {
+        const int32_t t = 4 * f;
+        tids[base[c.x - w.x] + atomicSub(&lc[c.x - w.x], 1) - 1] = t;
+        tids[base[c.y - w.x] + atomicSub(&lc[c.y - w.x], 1) - 1] = t + 1;
+        tids[base[c.z - w.x] + atomicSub(&lc[c.z - w.x], 1) - 1] = t + 2;
+        tids[base[c.w - w.x] + atomicSub(&lc[c.w - w.x], 1) - 1] = t + 3;
+      } else {
+        for (int32_t k = s; k < e; ++k) {
+          const int32_t i = ids[k] - w.x;
+          tids[base[i] + atomicSub(&lc[i], 1) - 1] = k;
+        }
+      }
+    }
+    return;
+  }
+  if (!act) return;
   if (QUAD) {
     int4 c = reinterpret_cast<const int4*>(ids)[f];
     int32_t t = 4 * f;
@@ -1701,10 +1839,27 @@ void launch_digest(const void* p, int64_t bytes, uint64_t* out, hipStream_t st) 
   k_digest<<<g, kBlock, 0, st>>>(static_cast<const uint8_t*>(p), bytes, reinterpret_cast<unsigned long long*>(out));
 }
 
+// the valence count and transpose through the block's vertex window (default);
+// SLAK_COUNT_WINDOW=0 takes one device atomic per corner (read once)
+static bool count_window() {
+  static const bool on = [] {
+    const char* e = std::getenv("SLAK_COUNT_WINDOW");
+    return !e || std::atoi(e) != 0;
+  }();
+  return on;
+}
+
 void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int32_t* fptr, const float4* P, int32_t nf,
                   const SegTable& seg, int32_t* cnt, const int* guard, hipStream_t st) {
   if (nf <= 0) return;
   unsigned g = grid_for(nf);
+  if (count && !fp && count_window()) {  // the valence count alone: through the block's vertex window
+    if (quad)
+      k_count<true><<<g, kBlock, 0, st>>>(ids, fptr, nf, cnt, guard);
+    else
+      k_count<false><<<g, kBlock, 0, st>>>(ids, fptr, nf, cnt, guard);
+    return;
+  }
 #define SLAK_FACES(Q, A, B) k_faces<Q, A, B><<<g, kBlock, 0, st>>>(ids, fptr, P, nf, seg, cnt, guard)
   if (quad) {
     if (fp && count) SLAK_FACES(true, true, true);
@@ -1721,10 +1876,11 @@ void launch_faces(bool quad, bool fp, bool count, const int32_t* ids, const int3
 void launch_transpose(bool quad, const int32_t* ids, const int32_t* fptr, int32_t nf, const int32_t* tptr,
                       int32_t* cnt, int32_t* tids, const int* guard, hipStream_t st) {
   if (nf <= 0) return;
+  const int window = count_window() ? 1 : 0;
   if (quad)
-    k_transpose<true><<<grid_for(nf), kBlock, 0, st>>>(ids, fptr, nf, tptr, cnt, tids, guard);
+    k_transpose<true><<<grid_for(nf), kBlock, 0, st>>>(ids, fptr, nf, tptr, cnt, tids, guard, window);
   else
-    k_transpose<false><<<grid_for(nf), kBlock, 0, st>>>(ids, fptr, nf, tptr, cnt, tids, guard);
+    k_transpose<false><<<grid_for(nf), kBlock, 0, st>>>(ids, fptr, nf, tptr, cnt, tids, guard, window);
 }
 
 // cap: M^T row length held in registers (4 or 8); longer rows take the

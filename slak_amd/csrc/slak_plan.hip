@@ -113,14 +113,15 @@ int autotune_cap(bool quad, int64_t nv, int64_t long_rows, int max_row) {
 // histogram (hist: kRowHistBins counts, the last bin "longer"; without one,
 // sigma = 1): R is the largest of 256, 128, 64 whose need fits kRowTileMax,
 // and the tile that need rounded up to 256 entries (at least 1024). A block
-// whose rows exceed the tile reads them from global memory. On levels of
-// 2^18 vertices or more the numbering takes half as many rows per block (at
-// least 64). Medians of interleaved runs on a 1024 x 1024 torus: numbering
-// 38.6 -> 35.0 us with quads, 89.5 -> 72.5 us with triangles, where the sizing
-// pass is fastest at R = 256 (33.3 against 43.5 us, 62.8 against 74.7); on
-// the 256 x 128 mixed torus (32768 vertices) the numbering is faster at 256
-// (11.8 against 13.5 us). SLAK_ROWS_PER_BLOCK forces the sizing pass's R,
-// SLAK_ROWS_NUMBER the numbering's.
+// whose rows exceed the tile reads them from global memory. With rows
+// averaging more than 5 the numbering takes half as many rows per block (at
+// least 64). Medians of interleaved runs on a 1024 x 1024 triangle torus
+// (rows of 6): numbering 89.5 -> 72.5 us, while the sizing pass is fastest at
+// R = 256 (62.8 against 74.7 us at 128). With rows of about 4 the numbering
+// gains little or loses at R/2 (quad 1024 x 1024 torus 38.6 -> 35.0 us, the
+// 256 x 128 mixed torus 11.8 -> 13.5 us, the 4096-mesh batch 385 -> 450 us).
+// SLAK_ROWS_PER_BLOCK forces the sizing pass's R, SLAK_ROWS_NUMBER the
+// numbering's.
 RowTiling autotune_rows(int64_t nv, int64_t nnz, const int64_t* hist) {
   double m = nv > 0 ? (double)nnz / (double)nv : 4.0;
   double sigma = 1.0;
@@ -145,7 +146,7 @@ RowTiling autotune_rows(int64_t nv, int64_t nnz, const int64_t* hist) {
   }
   const double want = std::ceil(need(R) / 256.0) * 256.0;
   const int tile = (int)std::min<double>(kRowTileMax, std::max(1024.0, want));
-  int Rn = nv >= (int64_t(1) << 18) ? std::max(64, R / 2) : R;
+  int Rn = m > 5.0 ? std::max(64, R / 2) : R;
   if (const char* e = std::getenv("SLAK_ROWS_NUMBER")) {
     const int forced = std::atoi(e);
     if (forced == 64 || forced == 128 || forced == 256) Rn = forced;
