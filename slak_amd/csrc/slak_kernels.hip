@@ -1697,6 +1697,9 @@ __global__ void __launch_bounds__(kScanSmallThreads) k_scan_small(const int32_t*
 // + j (r < kScanRounds, j < 4), so that every load and store instruction of a
 // wave covers 1 KB of consecutive ints.
 constexpr int kScanRounds = kScanItems / 4;
+// up to this many tiles (8 M items) the down pass sums the preceding tile
+// sums itself (at most 16 per lane) instead of a partials launch
+constexpr int64_t kScanFoldTiles = 4096;
 static_assert(kScanRounds * 4 * kBlock == kScanTile, "tile layout");
 
 __device__ __forceinline__ int4 scan_load4(const int32_t* __restrict__ in, int64_t i, int64_t n, bool vec) {
@@ -1740,10 +1743,38 @@ __global__ void __launch_bounds__(kBlock) k_scan_partials(int64_t* __restrict__ 
   }
 }
 
+// FOLD: partial holds the tile sums (no partials pass); each block adds up
+// the sums of the tiles before it, and block 0 also forms the total
+// (out[n], total_out, the overflow flag). Without FOLD partial holds the
+// exclusive tile prefixes.
+template <bool FOLD>
 __global__ void __launch_bounds__(kBlock) k_scan_down(const int32_t* __restrict__ in, int64_t n,
-                                                      const int64_t* __restrict__ partial, int32_t* __restrict__ out) {
+                                                      const int64_t* __restrict__ partial, int32_t* __restrict__ out,
+                                                      int64_t ntiles, int* __restrict__ flags,
+                                                      int32_t* __restrict__ total_out) {
   constexpr int kWaves = kBlock / 64;
   __shared__ int64_t wsum[kScanRounds * kWaves];
+  __shared__ int64_t red[kBlock / 64];
+  int64_t prefix = 0;
+  if (FOLD) {
+    int64_t acc = 0;
+    for (int64_t i = threadIdx.x; i < (int64_t)blockIdx.x; i += kBlock) acc += partial[i];
+    block_exclusive_scan(acc, red, prefix);
+    if (blockIdx.x == 0) {
+      int64_t all = 0, carry;
+      for (int64_t i = threadIdx.x; i < ntiles; i += kBlock) all += partial[i];
+      block_exclusive_scan(all, red, carry);
+      if (threadIdx.x == 0) {
+        if (carry > 0x7fffffffll) atomicOr(flags, kFlagOverflow);
+        out[n] = (int32_t)carry;
+        if (total_out) *total_out = (int32_t)carry;
+      }
+    }
+  } else {
+    (void)ntiles;
+    (void)flags;
+    (void)total_out;
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool vin = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
   const bool vout = (reinterpret_cast<uintptr_t>(out) & 15) == 0;
@@ -1762,7 +1793,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_down(const int32_t* __restrict_
     for (int r = 0; r < kScanRounds; ++r) wsum[r * kWaves + wave] = v[r];
   __syncthreads();
   if (threadIdx.x == 0) {  // the wave totals, rounds major: exclusive prefixes in place
-    int64_t run = partial[blockIdx.x];
+    int64_t run = FOLD ? prefix : partial[blockIdx.x];
     for (int k = 0; k < kScanRounds * kWaves; ++k) {
       const int64_t t = wsum[k];
       wsum[k] = run;
@@ -2129,8 +2160,12 @@ void launch_scan(const int32_t* in, int32_t* out, int64_t n, int64_t* partial, i
   }
   int64_t ntiles = (n + kScanTile - 1) / kScanTile;
   k_scan_reduce<<<(unsigned)ntiles, kBlock, 0, st>>>(in, n, partial);
+  if (ntiles <= kScanFoldTiles) {  // two launches: each tile adds up the sums before it
+    k_scan_down<true><<<(unsigned)ntiles, kBlock, 0, st>>>(in, n, partial, out, ntiles, flags, total_out);
+    return;
+  }
   k_scan_partials<<<1, kBlock, 0, st>>>(partial, ntiles, out, n, flags, total_out);
-  k_scan_down<<<(unsigned)ntiles, kBlock, 0, st>>>(in, n, partial, out);
+  k_scan_down<false><<<(unsigned)ntiles, kBlock, 0, st>>>(in, n, partial, out, ntiles, flags, total_out);
 }
 
 }  // namespace slak

@@ -152,10 +152,12 @@ def test_spgemm_operator_matches_oracle(oracle, make):
     assert r.nedges == d["nedges"] and r.nextern == d["nextern"]
 
 
-@pytest.mark.parametrize("n", [0, 1, 7, 2047, 2048, 2049, 8191, 8192, 8193, 32767, 32768, 32769, 100000, 3_000_001])
+@pytest.mark.parametrize("n", [0, 1, 7, 2047, 2048, 2049, 8191, 8192, 8193, 32767, 32768, 32769, 100000, 3_000_001,
+                               8_388_608, 9_000_001])
 def test_scan_operator(n):
-    """Exclusive scan with total; sizes around the one-block chunk (8192) and
-    the one-block / three-kernel threshold (32768)."""
+    """Exclusive scan with total; sizes around the one-block chunk (8192), the
+    one-block / tiled threshold (32768) and the tiled scan's two-launch /
+    three-launch threshold (4096 tiles of 2048 items)."""
     import ctypes
     from slak_amd.mesh import DeviceBuffer, to_host
     rng = np.random.default_rng(n)
