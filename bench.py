@@ -656,6 +656,18 @@ def main():
     }
 
     if not args.no_legs:
+        # the same batch in the other scheme (graph replays, no plain form)
+        other = "slak" if args.scheme == "lak" else "lak"
+        osch = capi.SLAK_SCHEME_SLAK if other == "slak" else capi.SLAK_SCHEME_LAK
+        O = batched_leg(spec, osch, args.steps, args.warmup, keep_results=0, plain=False)
+        line["batched_other_scheme"] = {
+            "scheme": other.upper(),
+            "value": round(O["faces"] * args.steps / O["secs"], 1),
+            "unit": "faces/s",
+            "ms_per_step": round(O["secs"] / args.steps * 1e3, 4),
+            "output_faces": O["faces"],
+            "roofline": stage_roofline(O["prof"], CFG4_STAGE),
+        }
         legs = {}
         for w in ("cfg2", "cfg3"):
             m = make_single(w)
