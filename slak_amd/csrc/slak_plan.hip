@@ -1153,15 +1153,22 @@ void Plan::record_level(int l) {
   {
     // algorithmic bytes of the fused launch, each array once: reads ids,
     // (edge, twin) pairs, positions, face points and the M^T rows; writes
-    // refined quads, edge and vertex points and, for a level that feeds
-    // another, its (edge, twin) pairs, M^T rows and offsets, boundary flags
-    // and (SLAK) M^T positions
-    const double reads = 4.0 * L.nnz + 8.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 4.0 * L.nv + 8.0 * L.nnz +
-                         (L.blk ? 4.0 * ne : 0.0) + poly_b + (next && scheme_ == kSLAK ? 4.0 * L.nnz : 0.0);
-    const double writes = 16.0 * L.nnz + 16.0 * ne + 16.0 * L.nv +
-                          (next ? 32.0 * L.nnz + 32.0 * L.nnz + 4.0 * (L.nv + L.nf + ne) +
-                                      (L.nbnd ? 4.0 * (L.nnz + 2.0 * ne) : 0.0) + (scheme_ == kSLAK ? 16.0 * L.nnz : 0.0)
-                                : 0.0);
+    // edge and vertex points, the refined quads of the meshes that keep them
+    // (not those whose next level is formed by k_final) and, for the meshes
+    // that feed another level, their (edge, twin) pairs, M^T rows and
+    // offsets, boundary flags and (SLAK) M^T positions
+    double reads = 4.0 * L.nnz + 8.0 * L.nnz + 16.0 * L.nv + 16.0 * L.nf + 4.0 * L.nv + 8.0 * L.nnz +
+                   (L.blk ? 4.0 * ne : 0.0) + poly_b;
+    double writes = 16.0 * ne + 16.0 * L.nv;
+    for (const SegDesc& S : L.segs) {
+      const double snnz = (double)S.nnz, sne = (double)S.ne;
+      if (S.dst_ids) writes += 16.0 * snnz;
+      if (next && S.next_cb >= 0) {
+        writes += 64.0 * snnz + 4.0 * ((double)S.nv + (double)S.nf + sne) + (L.nbnd ? 4.0 * (snnz + 2.0 * sne) : 0.0) +
+                  (scheme_ == kSLAK ? 16.0 * snnz : 0.0);
+        if (scheme_ == kSLAK) reads += 4.0 * snnz;
+      }
+    }
     ProfScope ps(st_, L.quad ? "refine_vertex_quad" : "refine_vertex_poly", reads + writes);
     launch_level(L.quad, scheme_, R, st_);
   }
