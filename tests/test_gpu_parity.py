@@ -311,3 +311,68 @@ def test_closed_form_levels_equal_level_by_level(scheme, case, env, monkeypatch)
     if scheme == LAK:
         np.testing.assert_array_equal(other.ptr, base.ptr)
         np.testing.assert_array_equal(other.vals, base.vals)
+
+
+def _poly(faces, nverts):
+    ptr = np.zeros(len(faces) + 1, np.int32)
+    ptr[1:] = np.cumsum([len(f) for f in faces])
+    ids = np.concatenate([np.asarray(f, np.int32) for f in faces]) if faces else np.zeros(0, np.int32)
+    vals = np.concatenate([np.arange(1, len(f) + 1, dtype=np.int32) for f in faces]) if faces else np.zeros(0, np.int32)
+    rng = np.random.default_rng(len(faces))
+    verts = np.concatenate([rng.random((nverts, 3), dtype=np.float32), np.ones((nverts, 1), np.float32)], axis=1)
+    return g.HostMesh(g.POLY, ptr, ids, vals, verts)
+
+
+@pytest.mark.parametrize("kind", ["non_manifold_edge", "two_corner_face", "repeated_vertex", "flipped_face"])
+def test_malformed_topology_rejected(kind):
+    """Inputs the reference leaves undefined are rejected with the invalid-mesh
+    status, and the library stays usable: an edge shared by three faces, a
+    face of two corners, a face that visits a vertex twice, and a face whose
+    orientation disagrees with its neighbour's (a directed edge used twice)."""
+    if kind == "non_manifold_edge":
+        bad = _poly([[0, 1, 2], [1, 0, 3], [0, 1, 4]], 5)
+    elif kind == "two_corner_face":
+        bad = _poly([[0, 1, 2, 3], [0, 1]], 4)
+    elif kind == "repeated_vertex":
+        bad = _poly([[0, 1, 2, 1], [0, 2, 3]], 4)
+    else:
+        bad = _poly([[0, 1, 2, 3], [2, 5, 4, 1]], 6)  # 1 -> 2 in both faces
+    for scheme in (LAK, SLAK):
+        with pytest.raises(SlakError) as ei:
+            subdivide_host(bad, 1, scheme)
+        assert ei.value.code == 2, (kind, str(ei.value))
+    ok, _ = subdivide_host(g.cube(), 1, LAK)
+    assert ok.nfaces == 24
+
+
+def test_empty_mesh():
+    """A mesh without faces refines to an empty mesh or is rejected with a
+    status; it never crashes the library."""
+    empty = _poly([], 0)
+    try:
+        out, _ = subdivide_host(empty, 1, LAK)
+        assert out.nfaces == 0
+    except SlakError as e:
+        assert e.code in (1, 2), str(e)
+    ok, _ = subdivide_host(g.cube(), 1, LAK)
+    assert ok.nfaces == 24
+
+
+def test_batch_with_an_invalid_mesh_reports_it():
+    """A batch holding one malformed mesh fails at creation with the
+    invalid-mesh status; the next batch of valid meshes works."""
+    from slak_amd.batch import Batch
+    from slak_amd.mesh import MeshInfo
+    hosts = [g.cube(), _poly([[0, 1, 2], [1, 0, 3], [0, 1, 4]], 5), g.torus(5, 4, seed=1)]
+    dev = [MeshInfo.from_host(h) for h in hosts]
+    try:
+        with pytest.raises(SlakError) as ei:
+            with Batch(dev, [2, 2, 2], LAK) as b:
+                b.run(use_graph=False)
+        assert ei.value.code == 2, str(ei.value)
+        with Batch([dev[0], dev[2]], [2, 2], LAK) as b:
+            b.run(use_graph=True)
+            assert b.result(0).nfaces == 6 * 16
+    finally:
+        for d in dev:
+            d.freeAndReset()
