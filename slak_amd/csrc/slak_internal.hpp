@@ -85,7 +85,6 @@ void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st);
 void launch_final(int scheme, const FinalView& V, bool ids, hipStream_t st);
 // LAK form (ptr, vals) of every segment of a level with dst_ptr set; nnz =
 // the level's union corners (= refined faces)
-void launch_lak_form(const SegTable& seg, int64_t nnz, hipStream_t st);
 // homogeneous-mesh conversions: per-face reorder of (ids, vals) by vals or by
 // ids (faces by ptr, or stride when ptr is null; 3 or 4 corners); fills of
 // stride ptr / 1-based positions
@@ -135,14 +134,6 @@ class Pool {
 
 Pool& pool();
 
-// A second stream of the calling thread on the current device (default
-// priority, non-blocking) with a fork and a join event, for work that runs
-// beside a plan's chain. Lives until the thread ends.
-struct AuxStream {
-  hipStream_t stream = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
-};
-AuxStream& aux_stream();
 
 // Allocations of one library call: released back to the pool after the
 // call's final stream synchronization, except buffers handed to the caller.

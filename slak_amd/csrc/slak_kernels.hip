@@ -779,6 +779,13 @@ __device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K,
   const bool final_level = S->dst_final != 0;  // the caller's output: not read again here
   if (final_level) {
     store_nt(S->dst_ids + 4 * lt, c.v - K.vb + K.off, epb + c.e1, fpid, epb + c.e0);
+    if (S->dst_ptr) {
+      // LAK form of refined face lt (createQuadColPtr / createQuadVals,
+      // LAKCatmullClarkKernels.cuh:396-431): ptr = 4 lt, vals 1..4
+      store_nt(S->dst_ptr + lt, 4 * lt);
+      if (lt == (int32_t)S->nnz - 1) store_nt(S->dst_ptr + S->nnz, 4 * (int32_t)S->nnz);
+      if (S->dst_vals) store_nt(S->dst_vals + 4 * (int64_t)lt, 1, 2, 3, 4);
+    }
   } else if (S->dst_ids) {  // null: the next level is computed without them (k_final)
     reinterpret_cast<int4*>(S->dst_ids)[lt] = make_int4(c.v - K.vb + K.off, epb + c.e1, fpid, epb + c.e0);
   }
@@ -1499,22 +1506,6 @@ __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int3
   }
 }
 
-// ---------------------------------------------------------------------------
-// LAK form of a final level: refined face r of a mesh has ptr[r] = 4r and
-// vals[4r .. 4r+3] = 1, 2, 3, 4 (createQuadColPtr / createQuadVals,
-// LAKCatmullClarkKernels.cuh:396-431); one thread per refined face (= corner
-// of the level's union).
-
-__global__ void __launch_bounds__(kBlock) k_lak_form(SegTable seg, int64_t nnz) {
-  const int64_t t = block_index() * blockDim.x + threadIdx.x;
-  if (t >= nnz) return;
-  const SegDesc* __restrict__ S = seg.d + seg_of_wave(seg.cbase, seg.n, t);
-  if (!S->dst_ptr) return;
-  const int32_t lt = (int32_t)(t - S->cb);
-  store_nt(S->dst_ptr + lt, 4 * lt);
-  if (lt == (int32_t)S->nnz - 1) S->dst_ptr[S->nnz] = 4 * (int32_t)S->nnz;
-  if (S->dst_vals) store_nt(S->dst_vals + 4 * (int64_t)lt, 1, 2, 3, 4);
-}
 
 // ---------------------------------------------------------------------------
 // reduced / unreduced conversions of homogeneous meshes (GPU::CSCMesh::reduce /
@@ -2096,10 +2087,6 @@ void launch_final(int scheme, const FinalView& V0, bool ids, hipStream_t st) {
 #undef SLAK_FINAL
 }
 
-void launch_lak_form(const SegTable& seg, int64_t nnz, hipStream_t st) {
-  if (nnz <= 0) return;
-  k_lak_form<<<grid_for(nnz), kBlock, 0, st>>>(seg, nnz);
-}
 
 void launch_sort_faces(bool by_vals, const int32_t* ptr, int32_t stride, int32_t nf, int64_t nnz, int32_t* ids,
                        int32_t* vals, int* flags, hipStream_t st) {

@@ -160,15 +160,6 @@ Plan::Plan(int scheme, std::vector<MeshJob> jobs, OutputMode mode, hipStream_t s
 }
 
 Plan::~Plan() {
-  // a launch that threw after the LAK forms were queued on the auxiliary
-  // stream: they read the level tables in the arena released below
-  if (lak_forms_pending_) {
-    try {
-      (void)hipStreamSynchronize(aux_stream().stream);
-    } catch (...) {
-      (void)hipDeviceSynchronize();
-    }
-  }
   if (exec_) (void)hipGraphExecDestroy(exec_);
   if (graph_) (void)hipGraphDestroy(graph_);
   if (owned_st_) {
@@ -778,7 +769,6 @@ void Plan::prepare() {
         S.dst_ids = J.out_ids;
         S.dst_ptr = J.out_ptr;
         S.dst_vals = J.out_vals;
-        L.lak_form |= J.out_ptr != nullptr;
         S.id_offset = 0;
         S.next_vb = -1;
         S.next_cb = -1;
@@ -1163,6 +1153,7 @@ void Plan::record_level(int l) {
     for (const SegDesc& S : L.segs) {
       const double snnz = (double)S.nnz, sne = (double)S.ne;
       if (S.dst_ids) writes += 16.0 * snnz;
+      if (S.dst_ptr) writes += 4.0 * (snnz + 1) + 16.0 * snnz;  // LAK form of a final level
       if (next && S.next_cb >= 0) {
         writes += 64.0 * snnz + 4.0 * ((double)S.nv + (double)S.nf + sne) + (L.nbnd ? 4.0 * (snnz + 2.0 * sne) : 0.0) +
                   (scheme_ == kSLAK ? 16.0 * snnz : 0.0);
@@ -1174,34 +1165,10 @@ void Plan::record_level(int l) {
   }
 }
 
-// The LAK form of a final level (ptr[t] = 4t, vals cyclic 1..4,
-// createQuadColPtr / createQuadVals, LAKCatmullClarkImpl.cu:553-565) depends
-// only on sizes: it is written on an auxiliary stream while the chain runs,
-// mostly during the small first levels that leave the GPU idle.
-void Plan::launch_lak_forms() {
-  bool any = false;
-  for (auto& L : levels_) any |= L.lak_form;
-  if (!any) return;
-  AuxStream& aux = aux_stream();
-  SLAK_HIP_CHECK(hipEventRecord(aux.fork, st_));
-  SLAK_HIP_CHECK(hipStreamWaitEvent(aux.stream, aux.fork, 0));
-  for (auto& L : levels_) {
-    if (!L.lak_form) continue;
-    double nnz_final = 0;
-    for (auto& S : L.segs)
-      if (S.dst_ptr) nnz_final += (double)S.nnz;
-    ProfScope ps(aux.stream, "lak_form", 20.0 * nnz_final + 4.0);
-    launch_lak_form(L.table(), L.nnz, aux.stream);
-  }
-  SLAK_HIP_CHECK(hipEventRecord(aux.join, aux.stream));
-  lak_forms_pending_ = true;
-}
-
 void Plan::launch(bool include_analysis) {
   if (levels_.empty()) return;
   if (include_analysis) number_early_ = false;  // the analysis runs again: number in order
   LevelPlan& L1 = levels_[0];
-  launch_lak_forms();  // on the second stream; a graph captures the fork and join
   if (include_analysis) {
     launch_fill_i32(d_state_, n_state_, 0, st_);  // a fill kernel: graph-safe
     {
@@ -1217,10 +1184,6 @@ void Plan::launch(bool include_analysis) {
     record_analysis(L1);
   }
   for (int l = 1; l <= (int)levels_.size(); ++l) record_level(l);
-  if (lak_forms_pending_) {
-    SLAK_HIP_CHECK(hipStreamWaitEvent(st_, aux_stream().join, 0));
-    lak_forms_pending_ = false;
-  }
 }
 
 void Plan::run_graph() {

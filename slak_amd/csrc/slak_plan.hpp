@@ -97,7 +97,6 @@ struct LevelPlan {
   std::vector<int> jobs;  // segment -> job
   std::vector<SegDesc> segs;
   std::vector<int64_t> vbase, fbase, cbase;
-  bool lak_form = false;  // some mesh finishes here with the LAK form (ptr / vals)
   // meshes whose last level is this one, formed by k_final (output quads,
   // LAK form, positions); they are not segments of this level's union
   FinalTable fin;
@@ -192,7 +191,6 @@ class Plan {
   hipStream_t capture_st_ = nullptr;  // stream the graph is captured on and replayed
   hipStream_t owned_st_ = nullptr;    // created when the caller's stream is the default one
   hipEvent_t order_ev_ = nullptr;
-  bool lak_forms_pending_ = false;
   // some mesh refined two or more levels with a quad mesh two levels before
   // its output: its last level is computed from that mesh's structure
   // (k_final) and the level before it writes positions only
@@ -211,7 +209,6 @@ class Plan {
   bool early_faces_ = false;
   float4* early_p1_ = nullptr;
   int64_t early_p1_cap_ = 0;
-  void launch_lak_forms();
   std::vector<void*> pooled_outputs_;
 };
 

@@ -100,32 +100,6 @@ void Pool::trim() {
   for (auto& kv : victims) (void)hipFree(kv.second);
 }
 
-AuxStream& aux_stream() {
-  struct Holder {
-    std::vector<AuxStream> per_dev;
-    ~Holder() {
-      for (auto& a : per_dev)
-        if (a.stream) {
-          (void)hipStreamSynchronize(a.stream);
-          (void)hipStreamDestroy(a.stream);
-          (void)hipEventDestroy(a.fork);
-          (void)hipEventDestroy(a.join);
-        }
-    }
-  };
-  thread_local Holder h;
-  int dev = 0;
-  SLAK_HIP_CHECK(hipGetDevice(&dev));
-  if ((int)h.per_dev.size() <= dev) h.per_dev.resize(dev + 1);
-  AuxStream& a = h.per_dev[dev];
-  if (!a.stream) {
-    SLAK_HIP_CHECK(hipStreamCreateWithFlags(&a.stream, hipStreamNonBlocking));
-    SLAK_HIP_CHECK(hipEventCreateWithFlags(&a.fork, hipEventDisableTiming));
-    SLAK_HIP_CHECK(hipEventCreateWithFlags(&a.join, hipEventDisableTiming));
-  }
-  return a;
-}
-
 Pool& pool() {
   static std::mutex mu;
   static std::vector<std::unique_ptr<Pool>> pools;
