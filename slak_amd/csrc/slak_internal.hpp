@@ -78,13 +78,12 @@ void launch_symbolic(bool quad, int scheme, int cap, const LevelView& L, hipStre
 // sorted in place for rows above the register capacity)
 void launch_number_faces(bool quad, int scheme, int cap, const LevelView& L, bool faces, hipStream_t st);
 void launch_spgemm_fde(bool quad, const LevelView& L, hipStream_t st);
-// refine (per corner) and vertex (per vertex) items of a level in one launch
+// refine (per corner) and vertex (per vertex) items of a level in one launch;
+// the refine items also write the LAK form of the segments with dst_ptr set
 void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st);
-// final level of a single mesh from the quad mesh two levels back
-// ids: also the output quads and LAK form (false: positions only)
+// last level of one mesh or several (V.segs) from the quad mesh two levels
+// back; ids: also the output quads and LAK form (false: positions only)
 void launch_final(int scheme, const FinalView& V, bool ids, hipStream_t st);
-// LAK form (ptr, vals) of every segment of a level with dst_ptr set; nnz =
-// the level's union corners (= refined faces)
 // homogeneous-mesh conversions: per-face reorder of (ids, vals) by vals or by
 // ids (faces by ptr, or stride when ptr is null; 3 or 4 corners); fills of
 // stride ptr / 1-based positions
