@@ -3,13 +3,14 @@
 
 Headline workload: the batched uneven workload, 4096 independent seeded tori
 (grid dims uniform in [4, 64]^2, 20 % of quads triangulated, depth uniform in
-{1..4}, about 443 M final faces). A step is one replay of the whole batched
+{1..4}, 437 M final faces). A step is one replay of the whole batched
 level chain (one launch per kernel per level over the union of the meshes
 still active at that level, captured once as a hipGraph) with the control
 cages resident on the device. Prints one JSON line with the throughput, the
 roofline of the dominant kernel (HIP events inside the library on the launch
 stream, HBM counters from rocprofv3), the plain per-mesh form for the
-speedup, secondary legs (the 256 x 128 mixed torus at 4 levels, the
+speedup, the same batch in the other scheme, secondary legs (the
+256 x 128 mixed torus at 4 levels, the
 1024 x 1024 torus at 2 levels with per-stage GB/s, the served stream) and
 the CPU baseline (the same LAK algorithm, FP32, on the host's cores) with the
 per-config position error against the FP64 oracle.
