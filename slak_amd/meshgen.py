@@ -504,3 +504,48 @@ def request_mesh(seed: int, index: int, faces_lo: float = 100.0, faces_hi: float
     lvl = level_lo + min(int(u[1] * span), span - 1)
     mesh = torus_mixed(a, b, p_tri, pentagons=False, seed=seed, mesh=index)
     return mesh, lvl
+
+
+def random_poly_torus(a: int, b: int, seed: int = 1, p_merge: float = 0.15, p_split: float = 0.2,
+                      p_hole: float = 0.3, sigma: float = 1e-3, mesh: int = 0) -> HostMesh:
+    """Torus grid (a x b quads, R = 1, r = 0.3) reworked at random: pairs of
+    quads along the major direction merged into hexagons (two corners of each
+    on a straight side), quads split into two triangles along a random
+    diagonal, and isolated holes (faces (i, j) with i, j = 1 mod 3 removed
+    with probability p_hole, so no two holes share a vertex and every
+    boundary vertex has one boundary fan). A mix of face orders 3, 4 and 6,
+    valences 2 to 8 and boundary loops. Decisions come from a NumPy
+    generator seeded with (seed, mesh)."""
+    if a < 3 or b < 3:
+        raise ValueError("random_poly_torus needs a, b >= 3")
+    rng = np.random.default_rng([seed, mesh, 7])
+    vid = lambda i, j: (j % b) * a + (i % a)  # noqa: E731
+    hole = np.zeros((a, b), bool)
+    for i in range(1, a - 1, 3):
+        for j in range(1, b - 1, 3):
+            hole[i, j] = rng.random() < p_hole
+    used = hole.copy()
+    faces = []
+    for j in range(b):
+        for i in range(a):
+            if used[i, j]:
+                continue
+            used[i, j] = True
+            q = [vid(i, j), vid(i + 1, j), vid(i + 1, j + 1), vid(i, j + 1)]
+            r = rng.random()
+            i2 = (i + 1) % a
+            if r < p_merge and not used[i2, j] and i2 != 0:
+                used[i2, j] = True
+                faces.append([vid(i, j), vid(i + 1, j), vid(i + 2, j), vid(i + 2, j + 1), vid(i + 1, j + 1),
+                              vid(i, j + 1)])
+            elif r < p_merge + p_split:
+                if rng.random() < 0.5:
+                    faces += [[q[0], q[1], q[2]], [q[0], q[2], q[3]]]
+                else:
+                    faces += [[q[0], q[1], q[3]], [q[1], q[2], q[3]]]
+            else:
+                faces.append(q)
+    u = 2.0 * np.pi * (np.arange(a * b) % a) / a
+    v = 2.0 * np.pi * (np.arange(a * b) // a) / b
+    pos = np.stack([(1.0 + 0.3 * np.cos(v)) * np.cos(u), (1.0 + 0.3 * np.cos(v)) * np.sin(u), 0.3 * np.sin(v)], 1)
+    return _mesh_from_faces(faces, pos, sigma, seed, mesh, f"random_poly_torus{a}x{b}_s{seed}")

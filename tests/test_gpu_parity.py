@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from slak_amd import capi, meshgen as g
-from slak_amd.mesh import MeshInfo, LAKCatmullClark, CatmullClark, subdivide_host, spgemm_mapped_host
+from slak_amd.mesh import MeshInfo, LAKCatmullClark, CatmullClark, subdivide_host, spgemm_mapped_host, to_host
 from slak_amd.capi import SlakError
 from tests import refimpl as R
 
@@ -373,6 +373,36 @@ def test_batch_with_an_invalid_mesh_reports_it():
         with Batch([dev[0], dev[2]], [2, 2], LAK) as b:
             b.run(use_graph=True)
             assert b.result(0).nfaces == 6 * 16
+    finally:
+        for d in dev:
+            d.freeAndReset()
+
+
+@pytest.mark.parametrize("scheme", [LAK, SLAK], ids=["LAK", "SLAK"])
+def test_random_poly_tori_vs_oracle(oracle, scheme):
+    """Randomly reworked tori (hexagons with straight sides, triangles,
+    isolated holes; valence 2..8, boundary loops beside polygons) at levels
+    1..3 against the oracle, then all of them in one batch with mixed depths
+    (graph replays) equal to their single-mesh results bit for bit."""
+    from slak_amd.batch import Batch
+    meshes = [g.random_poly_torus(12 + 3 * s, 8 + 2 * s, seed=s, p_merge=0.1 + 0.05 * (s % 3),
+                                  p_hole=0.6 if s % 2 else 0.0) for s in range(8)]
+    singles = []
+    for s, m in enumerate(meshes):
+        for levels in (1, 2, 3):
+            out, _, err = check_against_oracle(oracle, m, levels, scheme)
+        singles.append(subdivide_host(m, 1 + s % 4, scheme)[0])
+    dev = [MeshInfo.from_host(m) for m in meshes]
+    try:
+        with Batch(dev, [1 + s % 4 for s in range(len(meshes))], scheme) as b:
+            for _ in range(2):
+                b.run(use_graph=True)
+                for s, want in enumerate(singles):
+                    r = b.result(s)
+                    ids = to_host(r.ids, r.nnz, np.int32)
+                    verts = to_host(r.verts, 4 * r.nverts, np.float32).reshape(-1, 4)
+                    np.testing.assert_array_equal(ids, want.ids, err_msg=f"mesh {s}")
+                    np.testing.assert_array_equal(verts, want.verts, err_msg=f"mesh {s}")
     finally:
         for d in dev:
             d.freeAndReset()

@@ -53,3 +53,21 @@ def test_batch_and_request_generators():
         _directed_edges(m)
     m, lvl = g.request_mesh(5, 3)
     assert 1 <= lvl <= 3 and 100 * 0.5 <= m.nfaces
+
+
+def test_random_poly_torus_is_a_manifold_with_isolated_holes():
+    for seed in range(5):
+        m = g.random_poly_torus(15, 10, seed=seed, p_merge=0.2, p_hole=0.7)
+        orders = np.diff(m.ptr)
+        assert set(orders.tolist()) <= {3, 4, 6}
+        faces = [m.ids[m.ptr[f]:m.ptr[f + 1]].tolist() for f in range(m.nfaces)]
+        he = set()
+        for fv in faces:
+            for v, w in zip(fv, fv[1:] + fv[:1]):
+                assert (v, w) not in he, "directed edge used twice"
+                he.add((v, w))
+        boundary = [(v, w) for v, w in he if (w, v) not in he]
+        starts = [v for v, _ in boundary]
+        assert len(starts) == len(set(starts)), "a vertex on two boundary loops"
+        assert np.bincount(m.ids, minlength=m.nverts).min() >= 2
+    assert g.random_poly_torus(15, 10, seed=3).nfaces == g.random_poly_torus(15, 10, seed=3).nfaces

@@ -65,6 +65,13 @@ SMALL = [
     lambda: g.torus_mixed(5, 4, 0.5, pentagons=False, seed=3),
 ]
 
+# random rework of a torus grid: hexagons with straight sides, triangles,
+# isolated holes (valence 2..8, boundary loops beside polygons)
+RANDOM = [
+    lambda: g.random_poly_torus(9, 6, seed=3),
+    lambda: g.random_poly_torus(10, 7, seed=5, p_merge=0.3, p_hole=0.8),
+]
+
 # rows longer than the register capacities (valence 9..24) and faces of order 12
 HIGH_VALENCE = [
     lambda: g.uv_sphere(24, 4),
@@ -75,7 +82,7 @@ HIGH_VALENCE = [
 ]
 
 
-@pytest.mark.parametrize("make", SMALL + HIGH_VALENCE)
+@pytest.mark.parametrize("make", SMALL + HIGH_VALENCE + RANDOM)
 @pytest.mark.parametrize("scheme", [LAK, SLAK])
 def test_one_level_matches_textbook(oracle, make, scheme):
     m = make()
@@ -93,7 +100,7 @@ def test_one_level_matches_textbook(oracle, make, scheme):
         assert r.ptr is None and r.vals is None
 
 
-@pytest.mark.parametrize("make", SMALL[:6] + HIGH_VALENCE)
+@pytest.mark.parametrize("make", SMALL[:6] + HIGH_VALENCE + RANDOM)
 def test_spgemm_matches_dense_brute_force(oracle, make):
     m = make()
     faces = R.faces_of(m.ptr, m.ids)
@@ -154,7 +161,7 @@ def test_chain_equals_iterated_levels(oracle):
     np.testing.assert_array_equal(full.P.astype(np.float32), cur.verts)
 
 
-@pytest.mark.parametrize("make", SMALL + HIGH_VALENCE)
+@pytest.mark.parametrize("make", SMALL + HIGH_VALENCE + RANDOM)
 def test_lak_and_slak_agree_up_to_edge_permutation(oracle, make):
     m = make()
     lak = run(oracle, m, 2, LAK)
@@ -170,7 +177,7 @@ def test_lak_and_slak_agree_up_to_edge_permutation(oracle, make):
     assert np.abs(slak.P - lak.P[p]).max() < 1e-13
 
 
-@pytest.mark.parametrize("make", SMALL + HIGH_VALENCE)
+@pytest.mark.parametrize("make", SMALL + HIGH_VALENCE + RANDOM)
 def test_refined_topology_invariants(oracle, make):
     m = make()
     r = run(oracle, m, 2, LAK)
