@@ -536,12 +536,12 @@ struct FinalView {
   FinalSeg one;          // a single mesh (nseg == 1, no table)
   const FinalSeg* segs;  // several meshes: device table
   // several meshes: work units of one block each, (mesh, first mesh-local
-  // item, count, 0): corner units of consecutive corners of one mesh in
-  // mesh order; vertex units of consecutive vertices of one class of one
-  // mesh, each mesh's vertex classes interleaved in proportion
-  const int4* cunits;
-  const int4* vunits;
-  int32_t ncunits, nvunits;
+  // item, count, kind): kind 0 consecutive corners of one mesh, kind 1
+  // consecutive vertices of one class of one mesh. Mesh by mesh, the corner
+  // units and every vertex class's units are interleaved by their relative
+  // position, so a mesh's vertex items run beside its corner items.
+  const int4* units;
+  int32_t nunits;
   int32_t nseg;
   int64_t nnz, nv;       // corner and vertex items of the launch
   // the vertex items as up to kMaxVSeg runs [vseg[s], vseg[s+1]) of vnb[s]

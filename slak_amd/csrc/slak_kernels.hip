@@ -1418,91 +1418,88 @@ __device__ __forceinline__ void final_vertex(const FinalView& V, const FinalSeg&
 // then the face points and edge points of every earlier level), each numbered
 // in the order of the original faces it descends from, so that every run
 // advances over the same part of the surface as the corner blocks and the
-// positions they gather are fetched once for both. A corner block stages its
-// 16 KB of output quads in LDS and stores them with consecutive lanes on
-// consecutive 16-byte pieces.
+// positions they gather are fetched once for both. Several meshes take their
+// blocks from V.units, which interleaves each mesh's corner and vertex units
+// the same way. A corner block stages its 16 KB of output quads in LDS and
+// stores them with consecutive lanes on consecutive 16-byte pieces.
 template <int SCHEME, bool BND, bool MULTI, bool IDS>
 __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int32_t nbv) {
   const int64_t b = block_index<kRemapFinal>();
-  const int64_t T = (int64_t)nbc + nbv;
-  const int64_t before = b * nbv / T, after = (b + 1) * nbv / T;
-  if (after > before) {
-    if (MULTI) {  // work unit `before`: consecutive vertices of one class of one mesh
-      const int4 U = V.vunits[before];
-      if ((int)threadIdx.x < U.z) final_vertex<SCHEME, BND>(V, V.segs[U.x], U.y + (int32_t)threadIdx.x);
+  FinalSeg S;
+  int32_t t0, cnt;
+  if (MULTI) {  // work unit b (block-uniform): corners or vertices of one mesh
+    const int4 U = V.units[b];
+    S = V.segs[U.x];
+    if (U.w) {
+      if ((int)threadIdx.x < U.z) final_vertex<SCHEME, BND>(V, S, U.y + (int32_t)threadIdx.x);
       return;
     }
-    int64_t u = before, rest = nbv;
-    int s = 0;
-    for (; s + 1 < V.nvseg; ++s) {
-      const int64_t nb = V.vnb[s];
-      const int64_t lo = u * nb / rest, hi = (u + 1) * nb / rest;
-      if (hi > lo) {
-        u = lo;
-        break;
-      }
-      u -= lo;
-      rest -= nb;
-    }
-    const int64_t v = (int64_t)V.vseg[s] + u * blockDim.x + threadIdx.x;
-    if (v < V.vseg[s + 1]) final_vertex<SCHEME, BND>(V, V.one, (int32_t)v);
+    t0 = U.y;
+    cnt = U.z;
   } else {
-    // one mesh per corner block: several meshes take it from the block's
-    // corner unit (block-uniform, like the single mesh's descriptor)
-    const int64_t cblk = b - before;
-    FinalSeg S;
-    int32_t t0, cnt;
-    if (MULTI) {
-      const int4 U = V.cunits[cblk];
-      S = V.segs[U.x];
-      t0 = U.y;
-      cnt = U.z;
-    } else {
-      S = V.one;
-      t0 = (int32_t)(cblk * kBlock);
-      cnt = (int32_t)min((int64_t)kBlock, V.nnz - t0);
-    }
-    const int32_t t = t0 + (int32_t)threadIdx.x;
-    if (!IDS) {  // positions only
-      if ((int)threadIdx.x < cnt) final_corner<SCHEME, BND, false>(V, S, t);
+    const int64_t T = (int64_t)nbc + nbv;
+    const int64_t before = b * nbv / T, after = (b + 1) * nbv / T;
+    if (after > before) {
+      int64_t u = before, rest = nbv;
+      int s = 0;
+      for (; s + 1 < V.nvseg; ++s) {
+        const int64_t nb = V.vnb[s];
+        const int64_t lo = u * nb / rest, hi = (u + 1) * nb / rest;
+        if (hi > lo) {
+          u = lo;
+          break;
+        }
+        u -= lo;
+        rest -= nb;
+      }
+      const int64_t v = (int64_t)V.vseg[s] + u * blockDim.x + threadIdx.x;
+      if (v < V.vseg[s + 1]) final_vertex<SCHEME, BND>(V, V.one, (int32_t)v);
       return;
     }
-    __shared__ int4 stage[kStageStride * kBlock];
-    Quads4 q4{};
-    if ((int)threadIdx.x < cnt) q4 = final_corner<SCHEME, BND, true>(V, S, t);
-    if ((int)threadIdx.x < cnt) {
-      int4* __restrict__ o = stage + kStageStride * threadIdx.x;
-      o[0] = q4.a;
-      o[1] = q4.b;
-      o[2] = q4.c;
-      o[3] = q4.d;
+    S = V.one;
+    t0 = (int32_t)((b - before) * kBlock);
+    cnt = (int32_t)min((int64_t)kBlock, V.nnz - t0);
+  }
+  const int32_t t = t0 + (int32_t)threadIdx.x;
+  if (!IDS) {  // positions only
+    if ((int)threadIdx.x < cnt) final_corner<SCHEME, BND, false>(V, S, t);
+    return;
+  }
+  __shared__ int4 stage[kStageStride * kBlock];
+  Quads4 q4{};
+  if ((int)threadIdx.x < cnt) q4 = final_corner<SCHEME, BND, true>(V, S, t);
+  if ((int)threadIdx.x < cnt) {
+    int4* __restrict__ o = stage + kStageStride * threadIdx.x;
+    o[0] = q4.a;
+    o[1] = q4.b;
+    o[2] = q4.c;
+    o[3] = q4.d;
+  }
+  __syncthreads();
+  const int nq = 4 * cnt;
+  int32_t* __restrict__ dst = S.out_ids + 16 * (int64_t)t0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int i = j * kBlock + threadIdx.x;
+    if (i < nq) {
+      const int4 x = stage[kStageStride * (i >> 2) + (i & 3)];
+      store_nt(dst + 4 * i, x.x, x.y, x.z, x.w);
     }
-    __syncthreads();
-    const int nq = 4 * cnt;
-    int32_t* __restrict__ dst = S.out_ids + 16 * (int64_t)t0;
+  }
+  // LAK form of the block's 4 x 256 output quads (createQuadColPtr /
+  // createQuadVals): every int4 of vals is (1, 2, 3, 4) and int4 q of ptr
+  // is 16q + (0, 4, 8, 12), so consecutive lanes take consecutive pieces
+  if (S.out_vals) {
+    int32_t* __restrict__ vals = S.out_vals + 16 * (int64_t)t0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int i = j * kBlock + threadIdx.x;
-      if (i < nq) {
-        const int4 x = stage[kStageStride * (i >> 2) + (i & 3)];
-        store_nt(dst + 4 * i, x.x, x.y, x.z, x.w);
-      }
+      if (i < nq) store_nt(vals + 4 * i, 1, 2, 3, 4);
     }
-    // LAK form of the block's 4 x 256 output quads (createQuadColPtr /
-    // createQuadVals): every int4 of vals is (1, 2, 3, 4) and int4 q of ptr
-    // is 16q + (0, 4, 8, 12), so consecutive lanes take consecutive pieces
-    if (S.out_vals) {
-      int32_t* __restrict__ vals = S.out_vals + 16 * (int64_t)t0;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int i = j * kBlock + threadIdx.x;
-        if (i < nq) store_nt(vals + 4 * i, 1, 2, 3, 4);
-      }
-    }
-    if (S.out_ptr) {
-      if ((int)threadIdx.x < cnt) store_nt(S.out_ptr + 4 * (int64_t)t, 16 * t, 16 * t + 4, 16 * t + 8, 16 * t + 12);
-      if (t == S.nnz - 1) store_nt(S.out_ptr + 4 * (int64_t)S.nnz, 16 * S.nnz);
-    }
+  }
+  if (S.out_ptr) {
+    if ((int)threadIdx.x < cnt) store_nt(S.out_ptr + 4 * (int64_t)t, 16 * t, 16 * t + 4, 16 * t + 8, 16 * t + 12);
+    if (t == S.nnz - 1) store_nt(S.out_ptr + 4 * (int64_t)S.nnz, 16 * S.nnz);
   }
 }
 
@@ -2051,11 +2048,10 @@ void launch_final(int scheme, const FinalView& V0, bool ids, hipStream_t st) {
     nbv += V.vnb[s];
   }
   int64_t nbc = grid_for(V.nnz);
-  if (multi) {
-    if (!V.cunits || !V.vunits || V.ncunits <= 0 || V.nvunits <= 0)
-      throw Error(5, "final level: no work units");  // SLAK_ERR_INTERNAL
-    nbc = V.ncunits;
-    nbv = V.nvunits;
+  if (multi) {  // one block per work unit of V.units
+    if (!V.units || V.nunits <= 0) throw Error(5, "final level: no work units");  // SLAK_ERR_INTERNAL
+    nbc = V.nunits;
+    nbv = 0;
   }
   if ((nbc + nbv) * kBlock > 0xffffffffll) throw Error(1, "level too large for one launch");
   const unsigned g = (unsigned)(nbc + nbv);

@@ -217,9 +217,7 @@ void Plan::upload_tables(int l0, int l1, Staging& staging, int64_t state_ints) {
   std::vector<size_t> offs;
   // k_final tables of several meshes: FinalSeg[n], corner and vertex item bases
   auto table_bytes = [&](const FinalTable& T) -> size_t {
-    return final_table(T) ? align(T.segs.size() * sizeof(FinalSeg)) + align(T.cunits.size() * sizeof(int4)) +
-                                align(T.vunits.size() * sizeof(int4))
-                          : 0;
+    return final_table(T) ? align(T.segs.size() * sizeof(FinalSeg)) + align(T.units.size() * sizeof(int4)) : 0;
   };
   auto fin_bytes = [&](const LevelPlan& L) -> size_t { return table_bytes(L.fin) + table_bytes(L.pos); };
   for (int l = l0; l < l1; ++l) {
@@ -266,12 +264,9 @@ void Plan::upload_tables(int l0, int l1, Staging& staging, int64_t state_ints) {
       std::memcpy(staging.p + o, T->segs.data(), T->segs.size() * sizeof(FinalSeg));
       T->d_segs = reinterpret_cast<FinalSeg*>(d + o);
       o = align(o + T->segs.size() * sizeof(FinalSeg));
-      std::memcpy(staging.p + o, T->cunits.data(), T->cunits.size() * sizeof(int4));
-      T->d_cunits = reinterpret_cast<int4*>(d + o);
-      o = align(o + T->cunits.size() * sizeof(int4));
-      std::memcpy(staging.p + o, T->vunits.data(), T->vunits.size() * sizeof(int4));
-      T->d_vunits = reinterpret_cast<int4*>(d + o);
-      o = align(o + T->vunits.size() * sizeof(int4));
+      std::memcpy(staging.p + o, T->units.data(), T->units.size() * sizeof(int4));
+      T->d_units = reinterpret_cast<int4*>(d + o);
+      o = align(o + T->units.size() * sizeof(int4));
     }
   }
   SLAK_HIP_CHECK(hipMemcpyAsync(d, staging.p, total, hipMemcpyHostToDevice, st_));
@@ -956,21 +951,18 @@ static std::vector<int64_t> vertex_classes(const MeshJob& J, int l) {
   return cls;
 }
 
-// k_final over several meshes: work units of at most one block each. Corner
-// units: consecutive corners of one mesh, mesh by mesh. Vertex units:
-// consecutive vertices of one class of one mesh; mesh by mesh, the units of
-// all classes are interleaved by their relative position in their class, so
-// that every class advances over the surface together with the corner items
-// (each class is numbered in the order of the original faces).
+// k_final over several meshes: work units of at most one block each, mesh by
+// mesh. A mesh's corner units (consecutive corners) and the units of each of
+// its vertex classes (consecutive vertices of the class) are interleaved by
+// their relative position in their list, so that every class advances over
+// the surface together with the corner items (each class is numbered in the
+// order of the original faces) and the positions a mesh's vertex items gather
+// are still in the L2 its corner items filled.
 void Plan::build_final_units(int l, FinalTable& T) {
-  T.cunits.clear();
-  T.vunits.clear();
-  for (size_t i = 0; i < T.segs.size(); ++i)
-    for (int64_t c0 = 0; c0 < T.segs[i].nnz; c0 += kBlockItems)
-      T.cunits.push_back(make_int4((int)i, (int)c0, (int)std::min<int64_t>(kBlockItems, T.segs[i].nnz - c0), 0));
+  T.units.clear();
   struct Key {
     int64_t num, den;  // relative position (2j + 1) / (2 nch) of the chunk
-    int c;
+    int c;             // -1: corners, else the vertex class
     int64_t j;
   };
   std::vector<Key> keys;
@@ -979,6 +971,8 @@ void Plan::build_final_units(int l, FinalTable& T) {
     std::vector<int64_t> off(cls.size() + 1, 0);
     for (size_t c = 0; c < cls.size(); ++c) off[c + 1] = off[c] + cls[c];
     keys.clear();
+    const int64_t ncc = (T.segs[i].nnz + kBlockItems - 1) / kBlockItems;
+    for (int64_t j = 0; j < ncc; ++j) keys.push_back(Key{2 * j + 1, 2 * ncc, -1, j});
     for (size_t c = 0; c < cls.size(); ++c) {
       const int64_t nch = (cls[c] + kBlockItems - 1) / kBlockItems;
       for (int64_t j = 0; j < nch; ++j) keys.push_back(Key{2 * j + 1, 2 * nch, (int)c, j});
@@ -986,9 +980,14 @@ void Plan::build_final_units(int l, FinalTable& T) {
     std::stable_sort(keys.begin(), keys.end(),
                      [](const Key& a, const Key& b) { return a.num * b.den < b.num * a.den; });
     for (const Key& k : keys) {
-      const int64_t first = off[k.c] + k.j * kBlockItems;
-      const int64_t count = std::min<int64_t>(kBlockItems, cls[k.c] - k.j * kBlockItems);
-      T.vunits.push_back(make_int4((int)i, (int)first, (int)count, 0));
+      if (k.c < 0) {
+        const int64_t c0 = k.j * kBlockItems;
+        T.units.push_back(make_int4((int)i, (int)c0, (int)std::min<int64_t>(kBlockItems, T.segs[i].nnz - c0), 0));
+      } else {
+        const int64_t first = off[k.c] + k.j * kBlockItems;
+        const int64_t count = std::min<int64_t>(kBlockItems, cls[k.c] - k.j * kBlockItems);
+        T.units.push_back(make_int4((int)i, (int)first, (int)count, 1));
+      }
     }
   }
 }
@@ -1041,10 +1040,8 @@ void Plan::launch_final_table(int l, FinalTable& T, bool ids, const char* stage)
   V.nv = T.vb.back();
   if (final_table(T)) {
     V.segs = T.d_segs;
-    V.cunits = T.d_cunits;
-    V.vunits = T.d_vunits;
-    V.ncunits = (int32_t)T.cunits.size();
-    V.nvunits = (int32_t)T.vunits.size();
+    V.units = T.d_units;
+    V.nunits = (int32_t)T.units.size();
     V.nseg = (int32_t)T.segs.size();
   } else {
     V.one = T.segs[0];

@@ -49,10 +49,9 @@ struct FinalTable {
   std::vector<int> jobs;
   std::vector<FinalSeg> segs;
   std::vector<int64_t> cb, vb;  // corner and vertex item bases (n + 1)
-  std::vector<int4> cunits, vunits;
+  std::vector<int4> units;
   FinalSeg* d_segs = nullptr;
-  int4* d_cunits = nullptr;
-  int4* d_vunits = nullptr;
+  int4* d_units = nullptr;
   bool empty() const { return segs.empty(); }
 };
 
