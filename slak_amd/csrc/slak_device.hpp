@@ -77,6 +77,19 @@ struct SegTable {
   const int64_t* cbase;  // n + 1 corner bases
 };
 
+// A pointer read from device memory (a mesh table entry) is generic to the
+// compiler: its accesses become FLAT instructions, whose completion it does
+// not count in order, so every wait on one of them waits for all of the
+// wave's outstanding memory operations, stores included. Passing the pointer
+// through the global address space lets it emit global instructions (every
+// pointer in the tables is device memory).
+// (Through an integer: a direct generic -> global -> generic cast pair is
+// folded away before the address spaces are inferred.)
+template <class T>
+__device__ __forceinline__ T* to_global(T* p) {
+  return (T*)(__attribute__((address_space(1))) T*)(uintptr_t)p;
+}
+
 // Segment of a union index: the last k with base[k] <= x (empty segments skipped).
 __device__ __forceinline__ int seg_of(const int64_t* base, int n, int64_t x) {
   if (n <= 1) return 0;
