@@ -694,25 +694,40 @@ __device__ __forceinline__ void store_nt(float4* p, float4 q) {
   __builtin_nontemporal_store(__builtin_bit_cast(v4f_t, q), reinterpret_cast<v4f_t*>(p));
 }
 
-// Per-segment constants of a refine thread.
+// Per-segment constants of a refine thread, all read from the segment table
+// up front: a table read issued after a store would wait for that store.
 struct SegK {
-  const SegDesc* S;
   int32_t vb, fb, cb, snv, snf, snnz, emax, off;
-  bool next;
+  int64_t eb;
+  float4* dst_verts;
+  int32_t* dst_ids;
+  int32_t* dst_ptr;
+  int32_t* dst_vals;
+  int32_t ncb, nvb, neb;  // next level's union bases (ncb < 0: the mesh finishes here)
+  bool next, dst_final;
 };
 
 __device__ __forceinline__ SegK seg_consts(const SegTable& seg, int32_t f) {
   SegK K;
-  K.S = seg.d + seg_of_wave(seg.fbase, seg.n, f);
-  K.vb = (int32_t)K.S->vb;
-  K.fb = (int32_t)K.S->fb;
-  K.cb = (int32_t)K.S->cb;
-  K.snv = K.S->nv;
-  K.snf = K.S->nf;
-  K.snnz = (int32_t)K.S->nnz;
-  K.emax = K.S->ne > 0 ? (int32_t)K.S->ne - 1 : 0;
-  K.off = (int32_t)K.S->id_offset;
-  K.next = K.S->next_cb >= 0;
+  const SegDesc* __restrict__ S = seg.d + seg_of_wave(seg.fbase, seg.n, f);
+  K.vb = (int32_t)S->vb;
+  K.fb = (int32_t)S->fb;
+  K.cb = (int32_t)S->cb;
+  K.snv = S->nv;
+  K.snf = S->nf;
+  K.snnz = (int32_t)S->nnz;
+  K.emax = S->ne > 0 ? (int32_t)S->ne - 1 : 0;
+  K.off = (int32_t)S->id_offset;
+  K.eb = S->eb;
+  K.dst_verts = to_global(S->dst_verts);
+  K.dst_ids = to_global(S->dst_ids);
+  K.dst_ptr = to_global(S->dst_ptr);
+  K.dst_vals = to_global(S->dst_vals);
+  K.ncb = (int32_t)S->next_cb;
+  K.nvb = (int32_t)S->next_vb;
+  K.neb = (int32_t)S->next_eb;
+  K.next = S->next_cb >= 0;
+  K.dst_final = S->dst_final != 0;
   return K;
 }
 
@@ -757,9 +772,9 @@ __device__ __forceinline__ void corner_ranks(const RefineView& R, const SegK& K,
   c.bl1 = 0;
   c.bl0 = 0;
   if (R.blk) {
-    const int32_t bb = R.blk[K.S->eb];
-    c.bl1 = R.blk[K.S->eb + c.e1] - bb;
-    c.bl0 = R.blk[K.S->eb + c.e0] - bb;
+    const int32_t bb = R.blk[K.eb];
+    c.bl1 = R.blk[K.eb + c.e1] - bb;
+    c.bl0 = R.blk[K.eb + c.e0] - bb;
   }
   c.mp_t = 0;
   c.mp_s0 = 0;
@@ -773,21 +788,20 @@ __device__ __forceinline__ void corner_ranks(const RefineView& R, const SegK& K,
 // and, for a level that feeds another, the refined mesh's structure.
 template <bool QUAD, int SCHEME>
 __device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K, const CornerIn& c) {
-  const SegDesc* __restrict__ S = K.S;
   const int32_t lt = c.t - K.cb, lf = c.f - K.fb;
   const int32_t epb = K.snv + K.snf + K.off, fpid = K.snv + lf + K.off;
-  const bool final_level = S->dst_final != 0;  // the caller's output: not read again here
+  const bool final_level = K.dst_final;  // the caller's output: not read again here
   if (final_level) {
-    store_nt(to_global(S->dst_ids) + 4 * lt, c.v - K.vb + K.off, epb + c.e1, fpid, epb + c.e0);
-    if (S->dst_ptr) {
+    store_nt(K.dst_ids + 4 * lt, c.v - K.vb + K.off, epb + c.e1, fpid, epb + c.e0);
+    if (K.dst_ptr) {
       // LAK form of refined face lt (createQuadColPtr / createQuadVals,
       // LAKCatmullClarkKernels.cuh:396-431): ptr = 4 lt, vals 1..4
-      store_nt(to_global(S->dst_ptr) + lt, 4 * lt);
-      if (lt == (int32_t)S->nnz - 1) store_nt(to_global(S->dst_ptr) + S->nnz, 4 * (int32_t)S->nnz);
-      if (S->dst_vals) store_nt(to_global(S->dst_vals) + 4 * (int64_t)lt, 1, 2, 3, 4);
+      store_nt(K.dst_ptr + lt, 4 * lt);
+      if (lt == K.snnz - 1) store_nt(K.dst_ptr + K.snnz, 4 * K.snnz);
+      if (K.dst_vals) store_nt(K.dst_vals + 4 * (int64_t)lt, 1, 2, 3, 4);
     }
-  } else if (S->dst_ids) {  // null: the next level is computed without them (k_final)
-    reinterpret_cast<int4*>(to_global(S->dst_ids))[lt] = make_int4(c.v - K.vb + K.off, epb + c.e1, fpid, epb + c.e0);
+  } else if (K.dst_ids) {  // null: the next level is computed without them (k_final)
+    reinterpret_cast<int4*>(K.dst_ids)[lt] = make_int4(c.v - K.vb + K.off, epb + c.e1, fpid, epb + c.e0);
   }
   if (c.own) {
     float4 q;
@@ -795,7 +809,7 @@ __device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K,
       q = f4mul(f4add(f4add(f4add(c.plo, c.phi), c.fa), c.fz), 0.25f);
     else
       q = f4mul(f4add(c.plo, c.phi), 0.5f);
-    float4* const dv = to_global(S->dst_verts) + (K.snv + K.snf + c.e1);
+    float4* const dv = K.dst_verts + (K.snv + K.snf + c.e1);
     if (final_level) store_nt(dv, q);
     else *dv = q;
   }
@@ -803,7 +817,7 @@ __device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K,
 
   // ---- the refined mesh's structure
   const int32_t cb = K.cb, snnz = K.snnz, snv = K.snv, snf = K.snf;
-  const int32_t ncb = (int32_t)S->next_cb, nvb = (int32_t)S->next_vb;
+  const int32_t ncb = K.ncb, nvb = K.nvb;
   const int32_t c4 = ncb + 4 * lt;
   const int32_t tw0 = c.s1 >= 0 ? ncb + 4 * (c.n1 - cb) + 3 : -1;
   const int32_t tw1 = ncb + 4 * (c.tn - cb) + 2;
@@ -846,7 +860,7 @@ __device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K,
   else first = c.t == R.fptr[c.f];
   if (first) R.ntptr[nvb + snv + lf] = ncb + snnz + lt;
   int32_t* __restrict__ nbnd = R.nbnd;
-  const int32_t neb = (int32_t)S->next_eb;
+  const int32_t neb = K.neb;
   if (SCHEME == kSLAK && nbnd) nbnd[neb + snnz + lt] = 0;  // face point edges are interior
   if (c.own) {
     R.ntptr[nep + c.e1] = row1;
@@ -932,7 +946,7 @@ __device__ __forceinline__ void refine_item(const RefineView& R, int64_t t64) {
   c.u = R.ids[c.tp];
   corner_topology<QUAD>(R, K, c, R.ci[c.t], R.ci[c.tp]);
   const bool vlo = c.v < c.w;
-  const float4* __restrict__ fp = to_global(K.S->dst_verts) + K.snv;  // face points, by mesh-local face
+  const float4* __restrict__ fp = K.dst_verts + K.snv;  // face points, by mesh-local face
   if (c.own) {
     const float4 Pv = R.P[c.v], Pw = R.P[c.w];
     c.plo = vlo ? Pv : Pw;
@@ -958,16 +972,22 @@ constexpr int kVertexRegs = 4;
 template <bool QUAD, int SCHEME, bool NEXT, bool BND>
 __device__ __forceinline__ void vertex_item(const RefineView& R, int32_t v) {
   if (v >= R.nv) return;
+  // the segment's constants, all read before the first store (a table read
+  // issued after a store would wait for that store)
   const SegDesc* __restrict__ S = R.seg.d + seg_of_wave(R.seg.vbase, R.seg.n, v);
   const int32_t vb = (int32_t)S->vb, fb = (int32_t)S->fb, cb = (int32_t)S->cb;
-  const int32_t snv = S->nv;
+  const int32_t snv = S->nv, snf = S->nf;
+  const int32_t semax = S->ne > 0 ? (int32_t)S->ne - 1 : 0;
   float4* __restrict__ dst = to_global(S->dst_verts);
+  const bool dst_final = S->dst_final != 0;
+  const int32_t ncb = NEXT ? (int32_t)S->next_cb : -1, nvb = NEXT ? (int32_t)S->next_vb : 0;
+  const int32_t neb = NEXT ? (int32_t)S->next_eb : 0;
   const float4* __restrict__ fpts = dst + snv;  // face points, by mesh-local face
   const float4* __restrict__ P = R.P;
   const int2* __restrict__ ci = R.ci;
   // NEXT: some mesh of the level continues (R.ntptr set); BND: the level has a
   // boundary (R.blk set)
-  const bool next = NEXT && S->next_cb >= 0;
+  const bool next = NEXT && ncb >= 0;
   constexpr bool bnd_level = BND;
   constexpr bool need_prev = NEXT || BND;  // (edge, twin) of the in-half-edge
   constexpr bool need_own = BND;           // (edge, twin) of the out-half-edge
@@ -1027,21 +1047,19 @@ __device__ __forceinline__ void vertex_item(const RefineView& R, int32_t v) {
     for (int i = 0; i < n; ++i)
       if (ci[prevc(row[i].x)].y < 0) q = f4add(q, f4mul(P[row[i].y], 0.125f));
   }
-  if (!next && S->dst_final) {
+  if (!next && dst_final) {
     store_nt(dst + (v - vb), q);
     return;
   }
   dst[v - vb] = q;
   if (!next) return;
 
-  const int32_t ncb = (int32_t)S->next_cb, nvb = (int32_t)S->next_vb;
-  const int32_t nep = nvb + snv + S->nf;
-  const int32_t emax = S->ne > 0 ? (int32_t)S->ne - 1 : 0;
+  const int32_t nep = nvb + snv + snf;
+  const int32_t emax = semax;
   const int32_t pos = p0 - cb;  // mesh-local M^T storage position of the row
   R.ntptr[nvb + (v - vb)] = ncb + pos;
   int2* __restrict__ nrow = R.ntrow + ncb + pos;
   int32_t* __restrict__ nbnd = (SCHEME == kSLAK) ? R.nbnd : nullptr;
-  const int32_t neb = (int32_t)S->next_eb;
 #pragma unroll
   for (int i = 0; i < kVertexRegs; ++i)
     if (i < n) {
