@@ -256,17 +256,20 @@ __device__ void gather_face(const SegTable& seg, int32_t f, int32_t nf_total, in
       return;
     }
   }
+  // any other face: its first kGatherRegs corners are loaded before the
+  // first store (a load issued after a store would wait for it), the rest
+  // one by one
+  constexpr int kGatherRegs = 8;
+  const int32_t snv = S->nv;
   uint32_t mask = 0;
-  for (int64_t c = s; c < e; ++c) {
-    int32_t id = sid[c];
-    if (id < 0 || id >= S->nv) {
+  auto corner = [&](int64_t c, int32_t id, int32_t pos) __attribute__((always_inline)) {
+    if (id < 0 || id >= snv) {
       bad |= kFlagBadId;
       id = 0;
     }
-    int32_t pos = sval ? sval[c] : (int32_t)(c - s + 1);
     if (pos < 1 || pos > order) {
       bad |= kFlagBadVals;
-      continue;
+      return;
     }
     if (order <= 32) {
       uint32_t bit = 1u << (pos - 1);
@@ -278,7 +281,18 @@ __device__ void gather_face(const SegTable& seg, int32_t f, int32_t nf_total, in
     }
     ids_pos[cb + s + pos - 1] = vb + id;
     if (cface) cface[cb + c] = f;
-  }
+  };
+  int32_t rid[kGatherRegs], rpos[kGatherRegs];
+#pragma unroll
+  for (int i = 0; i < kGatherRegs; ++i)
+    if (i < order) {
+      rid[i] = sid[s + i];
+      rpos[i] = sval ? sval[s + i] : i + 1;
+    }
+#pragma unroll
+  for (int i = 0; i < kGatherRegs; ++i)
+    if (i < order) corner(s + i, rid[i], rpos[i]);
+  for (int64_t c = s + kGatherRegs; c < e; ++c) corner(c, sid[c], sval ? sval[c] : (int32_t)(c - s + 1));
   if (fptr) {
     fptr[f] = (int32_t)(cb + s);
     if (f == nf_total - 1) fptr[nf_total] = (int32_t)(cb + e);
@@ -532,7 +546,7 @@ __device__ void sort_row_in_place(int32_t* a, int n) {
 // the same way. Longer segments use global memory.
 extern __shared__ __align__(16) unsigned char row_tile_lds[];
 
-template <bool QUAD, int SCHEME, int CAP>
+template <bool QUAD, int SCHEME, int CAP, bool STAGED>
 __device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, int32_t* hist, const int32_t* rows_in,
                                              int2* rows_out, int32_t s0);
 
@@ -557,9 +571,14 @@ __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
   if (staged)
     for (int i = threadIdx.x; i < len; i += blockDim.x) tile_in[i] = L.tids[s0 + i];
   __syncthreads();
-  if ((int)threadIdx.x < R && v < L.nv && valid)
-    symbolic_row<QUAD, SCHEME, CAP>(L, v, L.hist_rows ? h : nullptr, staged ? tile_in : nullptr,
-                                    staged && L.trow ? tile_out : nullptr, s0);
+  // (staged or not as a template argument: a pointer that may address LDS
+  // or global memory makes every access through it a FLAT instruction)
+  if ((int)threadIdx.x < R && v < L.nv && valid) {
+    if (staged)
+      symbolic_row<QUAD, SCHEME, CAP, true>(L, v, L.hist_rows ? h : nullptr, tile_in, L.trow ? tile_out : nullptr, s0);
+    else
+      symbolic_row<QUAD, SCHEME, CAP, false>(L, v, L.hist_rows ? h : nullptr, nullptr, nullptr, s0);
+  }
   __syncthreads();
   if (staged && L.trow)
     for (int i = threadIdx.x; i < len; i += blockDim.x) L.trow[s0 + i] = tile_out[i];
@@ -568,8 +587,9 @@ __global__ void __launch_bounds__(kBlock) k_symbolic(LevelView L) {
 
 // mapped-SpGEMM row sizing of vertex v (edges it owns), validation, the
 // sorted row with in-neighbours; row length into the block histogram.
-// rows_in / rows_out: the block's LDS tiles from position s0, or null
-template <bool QUAD, int SCHEME, int CAP>
+// STAGED: rows_in / rows_out are the block's LDS tiles from position s0
+// (rows_out null: no sorted rows wanted); else both null
+template <bool QUAD, int SCHEME, int CAP, bool STAGED>
 __device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, int32_t* hist, const int32_t* rows_in,
                                              int2* rows_out, int32_t s0) {
   int32_t s = L.tptr[v];
@@ -579,14 +599,18 @@ __device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, int3
   int bad = 0;
   // the sorted row goes to the block's LDS tile (or straight to trow)
   auto put = [&](int i, int2 x) __attribute__((always_inline)) {
-    if (rows_out)
-      rows_out[s - s0 + i] = x;
-    else if (L.trow)
+    if (STAGED) {
+      if (rows_out) rows_out[s - s0 + i] = x;
+    } else if (L.trow) {
       L.trow[s + i] = x;
+    }
   };
   if (n <= CAP) {
     RegRow<CAP, QUAD> r;
-    r.load(L, rows_in ? rows_in + (s - s0) : L.tids + s, n);
+    if (STAGED)
+      r.load(L, rows_in + (s - s0), n);
+    else
+      r.load(L, L.tids + s, n);
     row_symbolic<SCHEME>(r, v, count, bcount, bad);
 #pragma unroll
     for (int i = 0; i < CAP; ++i)
@@ -611,7 +635,7 @@ __device__ __forceinline__ void symbolic_row(const LevelView& L, int32_t v, int3
   }
 }
 
-template <bool QUAD, int SCHEME, int CAP>
+template <bool QUAD, int SCHEME, int CAP, bool STAGED>
 __device__ __forceinline__ void number_row(const LevelView& L, int32_t v, const int2* rows, int32_t s0);
 template <bool QUAD, bool FP, bool COUNT>
 __device__ __forceinline__ void faces_item(const int32_t* __restrict__ ids, const int32_t* __restrict__ fptr,
@@ -637,21 +661,30 @@ __global__ void __launch_bounds__(kBlock) k_number_faces(LevelView L, int32_t nb
     if (staged)
       for (int i = threadIdx.x; i < len; i += blockDim.x) tile[i] = L.trow[s0 + i];
     __syncthreads();
-    if ((int)threadIdx.x < R) number_row<QUAD, SCHEME, CAP>(L, (int32_t)(v0 + threadIdx.x), staged ? tile : nullptr, s0);
+    if ((int)threadIdx.x < R) {
+      if (staged)
+        number_row<QUAD, SCHEME, CAP, true>(L, (int32_t)(v0 + threadIdx.x), tile, s0);
+      else
+        number_row<QUAD, SCHEME, CAP, false>(L, (int32_t)(v0 + threadIdx.x), nullptr, s0);
+    }
   } else
     faces_item<QUAD, true, false>(L.ids, L.fptr, L.P, L.nf, L.seg, nullptr, nullptr,
                                   (int32_t)((b - nbn) * blockDim.x + threadIdx.x));
 }
 
-// rows: the block's LDS copy of trow from position s0, or null
-template <bool QUAD, int SCHEME, int CAP>
+// STAGED: rows is the block's LDS copy of trow from position s0 (else null)
+template <bool QUAD, int SCHEME, int CAP, bool STAGED>
 __device__ __forceinline__ void number_row(const LevelView& L, int32_t v, const int2* rows, int32_t s0) {
   if (v >= L.nv) return;
   int32_t s = L.tptr[v];
   int n = L.tptr[v + 1] - s;
   if (n <= CAP) {
     RegRow<CAP, QUAD> r;
-    r.load_sorted(L, rows ? rows + (s - s0) : L.trow + s, n);  // the symbolic pass's sorted rows
+    // the symbolic pass's sorted rows
+    if (STAGED)
+      r.load_sorted(L, rows + (s - s0), n);
+    else
+      r.load_sorted(L, L.trow + s, n);
     row_number<SCHEME>(r, L, v);
   } else {
     // the symbolic pass sorted every row longer than its own capacity; the
