@@ -1432,9 +1432,22 @@ __device__ __forceinline__ void final_vertex(const FinalView& V, const FinalSeg&
     }
 #pragma unroll
   for (int i = 0; i < kVertexRegs; ++i)
+    if (i < n) b[i] = P1[ep0 + clampi(cp[i].x, emax)];
+  // ep(e1(t_i)) is ep(e0) of the face across e1(t_i), another face of the
+  // row (but on the boundary): taken from b when an in-edge of the row has
+  // the same edge id (the same position, so the same bits), else loaded
+#pragma unroll
+  for (int i = 0; i < kVertexRegs; ++i)
     if (i < n) {
-      b[i] = P1[ep0 + clampi(cp[i].x, emax)];
-      a[i] = fp_of(tt[i], co[i], b[i]);
+      int m = -1;
+#pragma unroll
+      for (int j = 0; j < kVertexRegs; ++j)
+        if (j < n && cp[j].x == co[i].x) m = j;
+      float4 e1 = b[0];
+#pragma unroll
+      for (int j = 1; j < kVertexRegs; ++j) e1 = f4sel(m == j, b[j], e1);
+      if (m < 0) e1 = P1[ep0 + clampi(co[i].x, emax)];
+      a[i] = face_point(Pv, e1, P1[S.nv + (tt[i] >> 2)], b[i]);
     }
   bool boundary = false;
   if (BND) {
