@@ -727,6 +727,33 @@ __device__ __forceinline__ void store_nt(float4* p, float4 q) {
   __builtin_nontemporal_store(__builtin_bit_cast(v4f_t, q), reinterpret_cast<v4f_t*>(p));
 }
 
+// DPP quad permutations (the four lanes of a quad hold the corners of one
+// quad face): every lane of the quad reads lane K / lane i+1 / lane i-1.
+template <int K>
+__device__ __forceinline__ float4 quad_lane(float4 x) {
+  constexpr int ctrl = K * 0x55;
+  return make_float4(__int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.x), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.y), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.z), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.w), ctrl, 0xF, 0xF, false)));
+}
+__device__ __forceinline__ float4 quad_next(float4 x) {
+  constexpr int ctrl = 0x39;  // [1, 2, 3, 0]
+  return make_float4(__int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.x), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.y), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.z), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.w), ctrl, 0xF, 0xF, false)));
+}
+__device__ __forceinline__ float4 quad_prev(float4 x) {
+  constexpr int ctrl = 0x93;  // [3, 0, 1, 2]
+  return make_float4(__int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.x), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.y), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.z), ctrl, 0xF, 0xF, false)),
+                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.w), ctrl, 0xF, 0xF, false)));
+}
+__device__ __forceinline__ int32_t quad_next_i(int32_t x) { return __builtin_amdgcn_mov_dpp(x, 0x39, 0xF, 0xF, false); }
+__device__ __forceinline__ int32_t quad_prev_i(int32_t x) { return __builtin_amdgcn_mov_dpp(x, 0x93, 0xF, 0xF, false); }
+
 // Per-segment constants of a refine thread, all read from the segment table
 // up front: a table read issued after a store would wait for that store.
 struct SegK {
@@ -975,9 +1002,18 @@ __device__ __forceinline__ void refine_item(const RefineView& R, int64_t t64) {
   const SegK K = seg_consts(R.seg, c.f);
   if (R.ntptr && t64 == R.nnz - 1) R.ntptr[R.next_nv] = R.next_nnz;  // end of the next level's M^T
   c.v = R.ids[c.t];
-  c.w = R.ids[c.tn];
-  c.u = R.ids[c.tp];
-  corner_topology<QUAD>(R, K, c, R.ci[c.t], R.ci[c.tp]);
+  const int2 c1 = R.ci[c.t];
+  if (QUAD) {
+    // the face's other corners are the quad's other lanes (nnz = 4 nf, and
+    // blocks start on a face): w, u and the in-half-edge by DPP, not loads
+    c.w = quad_next_i(c.v);
+    c.u = quad_prev_i(c.v);
+    corner_topology<QUAD>(R, K, c, c1, make_int2(quad_prev_i(c1.x), quad_prev_i(c1.y)));
+  } else {
+    c.w = R.ids[c.tn];
+    c.u = R.ids[c.tp];
+    corner_topology<QUAD>(R, K, c, c1, R.ci[c.tp]);
+  }
   const bool vlo = c.v < c.w;
   const float4* __restrict__ fp = K.dst_verts + K.snv;  // face points, by mesh-local face
   if (c.own) {
@@ -1162,31 +1198,6 @@ __global__ void __launch_bounds__(kBlock) k_level(RefineView R, int32_t nbr, int
 // vertices of M. Every sum adds the same terms in the same order as the
 // refine and vertex items of M1 would: the results are the same bits.
 
-// DPP quad permutations: every lane of the quad reads lane K / lane i+1.
-template <int K>
-__device__ __forceinline__ float4 quad_lane(float4 x) {
-  constexpr int ctrl = K * 0x55;
-  return make_float4(__int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.x), ctrl, 0xF, 0xF, false)),
-                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.y), ctrl, 0xF, 0xF, false)),
-                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.z), ctrl, 0xF, 0xF, false)),
-                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.w), ctrl, 0xF, 0xF, false)));
-}
-__device__ __forceinline__ float4 quad_next(float4 x) {
-  constexpr int ctrl = 0x39;  // [1, 2, 3, 0]
-  return make_float4(__int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.x), ctrl, 0xF, 0xF, false)),
-                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.y), ctrl, 0xF, 0xF, false)),
-                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.z), ctrl, 0xF, 0xF, false)),
-                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.w), ctrl, 0xF, 0xF, false)));
-}
-__device__ __forceinline__ float4 quad_prev(float4 x) {
-  constexpr int ctrl = 0x93;  // [3, 0, 1, 2]
-  return make_float4(__int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.x), ctrl, 0xF, 0xF, false)),
-                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.y), ctrl, 0xF, 0xF, false)),
-                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.z), ctrl, 0xF, 0xF, false)),
-                     __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x.w), ctrl, 0xF, 0xF, false)));
-}
-__device__ __forceinline__ int32_t quad_next_i(int32_t x) { return __builtin_amdgcn_mov_dpp(x, 0x39, 0xF, 0xF, false); }
-__device__ __forceinline__ int32_t quad_prev_i(int32_t x) { return __builtin_amdgcn_mov_dpp(x, 0x93, 0xF, 0xF, false); }
 __device__ __forceinline__ float4 f4sel(bool c, float4 a, float4 b) { return c ? a : b; }
 // face point of a quad from its corners' positions in corner order (the sum
 // and division k_faces uses)
