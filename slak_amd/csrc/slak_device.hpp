@@ -512,6 +512,7 @@ struct RefineView {
   int32_t* nmpos;        // next level's M^T positions (SLAK)
   int32_t next_nv;       // next level's union vertices and corners (end of its M^T)
   int32_t next_nnz;
+  int32_t next_rows_x;   // 1: the next level's M^T rows are corners only (int32 in ntrow)
   // one mesh: its vertex items as up to kMaxVSeg class runs [vseg[s],
   // vseg[s+1]) of vnb[s] blocks, each run spread in proportion over the
   // launch (nvseg 0: one run)
@@ -544,6 +545,7 @@ struct FinalView {
   const int2* ci;        // (mesh-local edge id, twin corner or -1) per corner
   const int32_t* tptr;   // M^T row offsets
   const int2* trow;      // M^T rows (corner, in-neighbour), sorted
+  const int32_t* trow_x; // or the rows' corners alone (int32), or null
   const int32_t* blk;    // exclusive scan of the boundary edge flags, or null (closed)
   const int32_t* mpos;   // M^T position per corner (SLAK)
   FinalSeg one;          // a single mesh (nseg == 1, no table)

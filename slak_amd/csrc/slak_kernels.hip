@@ -754,6 +754,15 @@ __device__ __forceinline__ float4 quad_prev(float4 x) {
 __device__ __forceinline__ int32_t quad_next_i(int32_t x) { return __builtin_amdgcn_mov_dpp(x, 0x39, 0xF, 0xF, false); }
 __device__ __forceinline__ int32_t quad_prev_i(int32_t x) { return __builtin_amdgcn_mov_dpp(x, 0x93, 0xF, 0xF, false); }
 
+// entry k of the next level's M^T rows: (corner, in-neighbour), or the corner
+// alone when those rows are read by the last level's kernel only
+__device__ __forceinline__ void put_next_row(const RefineView& R, int64_t k, int2 x) {
+  if (R.next_rows_x)
+    reinterpret_cast<int32_t*>(R.ntrow)[k] = x.x;
+  else
+    R.ntrow[k] = x;
+}
+
 // Per-segment constants of a refine thread, all read from the segment table
 // up front: a table read issued after a store would wait for that store.
 struct SegK {
@@ -914,7 +923,7 @@ __device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K,
   // face point row: corner 2 of each refined quad of f, in corner order;
   // in-neighbour ep(e1)
   const int32_t nep = nvb + snv + snf;
-  R.ntrow[ncb + snnz + lt] = make_int2(c4 + 2, nep + c.e1);
+  put_next_row(R, ncb + snnz + lt, make_int2(c4 + 2, nep + c.e1));
   bool first;
   if (QUAD) first = (c.t & 3) == 0;
   else first = c.t == R.fptr[c.f];
@@ -931,10 +940,10 @@ __device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K,
       a0 = a1;
       a1 = x;
     }
-    int2* __restrict__ q = R.ntrow + row1;
+    auto q = [&](int k, int2 x) __attribute__((always_inline)) { put_next_row(R, row1 + k, x); };
     if (c.s1 < 0) {
-      q[0] = a0;
-      q[1] = a1;
+      q(0, a0);
+      q(1, a1);
     } else {
       int2 b0 = make_int2(ncb + 4 * (c.s1 - cb) + 1, nvb + (c.w - K.vb));
       int2 b1 = make_int2(ncb + 4 * (c.n1 - cb) + 3, nvb + snv + (c.g1 - K.fb));
@@ -948,10 +957,10 @@ __device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K,
       const int2 m0 = alo ? a0 : b0, m3 = ahi ? a1 : b1;
       const int2 l = alo ? b0 : a0, h = ahi ? b1 : a1;
       const bool lh = l.x < h.x;
-      q[0] = m0;
-      q[1] = lh ? l : h;
-      q[2] = lh ? h : l;
-      q[3] = m3;
+      q(0, m0);
+      q(1, lh ? l : h);
+      q(2, lh ? h : l);
+      q(3, m3);
     }
     if (nbnd) {
       // the refined mesh has snnz + 2 ne edges
@@ -1127,17 +1136,17 @@ __device__ __forceinline__ void vertex_item(const RefineView& R, int32_t v) {
   const int32_t emax = semax;
   const int32_t pos = p0 - cb;  // mesh-local M^T storage position of the row
   R.ntptr[nvb + (v - vb)] = ncb + pos;
-  int2* __restrict__ nrow = R.ntrow + ncb + pos;
+  const int64_t nrow = (int64_t)ncb + pos;  // first entry of the refined row
   int32_t* __restrict__ nbnd = (SCHEME == kSLAK) ? R.nbnd : nullptr;
 #pragma unroll
   for (int i = 0; i < kVertexRegs; ++i)
     if (i < n) {
-      nrow[i] = make_int2(ncb + 4 * (e[i].x - cb), nep + clampi(cp[i].x, emax));
+      put_next_row(R, nrow + i, make_int2(ncb + 4 * (e[i].x - cb), nep + clampi(cp[i].x, emax)));
       if (nbnd) nbnd[neb + pos + i] = co[i].y < 0 ? 1 : 0;
     }
   for (int i = kVertexRegs; i < n; ++i) {
     const int32_t t = row[i].x;
-    nrow[i] = make_int2(ncb + 4 * (t - cb), nep + clampi(ci[prevc(t)].x, emax));
+    put_next_row(R, nrow + i, make_int2(ncb + 4 * (t - cb), nep + clampi(ci[prevc(t)].x, emax)));
     if (nbnd) nbnd[neb + pos + i] = ci[t].y < 0 ? 1 : 0;
   }
 }
@@ -1422,7 +1431,10 @@ __device__ __forceinline__ void final_vertex(const FinalView& V, const FinalSeg&
   const int2* __restrict__ ci = V.ci + cb;  // mesh-local corners from here on
   const int32_t p0 = V.tptr[S.vb + v];
   const int n = V.tptr[S.vb + v + 1] - p0;
-  const int2* __restrict__ row = V.trow + p0;  // union corners
+  // the row's corners (union corners): (corner, in-neighbour) pairs, or
+  // corners alone
+  const int32_t* __restrict__ rx = V.trow_x ? V.trow_x + p0 : reinterpret_cast<const int32_t*>(V.trow + p0);
+  const int rs = V.trow_x ? 1 : 2;
   const float4 Pv = P1[v];
   auto prevc = [](int32_t t) __attribute__((always_inline)) { return (t & ~3) | ((t + 3) & 3); };
   // face point of M1 face t = (v, ep(e1(t)), fp(face of t), ep(e0(t)))
@@ -1434,7 +1446,7 @@ __device__ __forceinline__ void final_vertex(const FinalView& V, const FinalSeg&
   float4 a[kVertexRegs], b[kVertexRegs];
 #pragma unroll
   for (int i = 0; i < kVertexRegs; ++i)
-    if (i < n) tt[i] = row[i].x - cb;
+    if (i < n) tt[i] = rx[rs * i] - cb;
 #pragma unroll
   for (int i = 0; i < kVertexRegs; ++i)
     if (i < n) {
@@ -1466,7 +1478,7 @@ __device__ __forceinline__ void final_vertex(const FinalView& V, const FinalSeg&
     for (int i = 0; i < kVertexRegs; ++i)
       if (i < n) boundary |= co[i].y < 0 || cp[i].y < 0;
     for (int i = kVertexRegs; i < n; ++i) {
-      const int32_t t = row[i].x - cb;
+      const int32_t t = rx[rs * i] - cb;
       boundary |= ci[t].y < 0 || ci[prevc(t)].y < 0;
     }
   }
@@ -1477,23 +1489,23 @@ __device__ __forceinline__ void final_vertex(const FinalView& V, const FinalSeg&
     for (int i = 0; i < kVertexRegs; ++i)
       if (i < n) s = f4add(s, a[i]);
     for (int i = kVertexRegs; i < n; ++i) {
-      const int32_t t = row[i].x - cb;
+      const int32_t t = rx[rs * i] - cb;
       s = f4add(s, fp_of(t, ci[t], P1[ep0 + clampi(ci[prevc(t)].x, emax)]));
     }
 #pragma unroll
     for (int i = 0; i < kVertexRegs; ++i)
       if (i < n) s = f4add(s, b[i]);
-    for (int i = kVertexRegs; i < n; ++i) s = f4add(s, P1[ep0 + clampi(ci[prevc(row[i].x - cb)].x, emax)]);
+    for (int i = kVertexRegs; i < n; ++i) s = f4add(s, P1[ep0 + clampi(ci[prevc(rx[rs * i] - cb)].x, emax)]);
     const float nf = (float)n;
     q = f4div(f4add(f4div(s, nf), f4mul(Pv, nf - 2.0f)), nf);
   } else {
     q = f4mul(Pv, 0.75f);
     for (int i = 0; i < n; ++i) {
-      const int2 c = ci[row[i].x - cb];
+      const int2 c = ci[rx[rs * i] - cb];
       if (c.y < 0) q = f4add(q, f4mul(P1[ep0 + clampi(c.x, emax)], 0.125f));
     }
     for (int i = 0; i < n; ++i) {
-      const int2 c = ci[prevc(row[i].x - cb)];
+      const int2 c = ci[prevc(rx[rs * i] - cb)];
       if (c.y < 0) q = f4add(q, f4mul(P1[ep0 + clampi(c.x, emax)], 0.125f));
     }
   }

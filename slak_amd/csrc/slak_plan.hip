@@ -902,6 +902,10 @@ void Plan::prepare() {
       if (final_table(L.pos)) build_final_units(l, L.pos);
     }
   }
+  // a level whose meshes all come from k_final runs no refine or vertex
+  // items over its structure: only k_final reads its M^T rows, and it needs
+  // their corners alone
+  for (size_t i = 1; i < levels_.size(); ++i) levels_[i].rows_x = levels_[i].segs.empty() || levels_[i].pos_fused;
   upload_tables(0, (int)levels_.size(), staging_all_);
   prepared_ = true;
 }
@@ -1034,6 +1038,7 @@ void Plan::launch_final_table(int l, FinalTable& T, bool ids, const char* stage)
   V.ci = M.ci;
   V.tptr = M.tptr;
   V.trow = M.trow;
+  V.trow_x = M.rows_x ? reinterpret_cast<const int32_t*>(M.trow) : nullptr;
   V.blk = M.blk;
   V.mpos = M.mpos;
   V.nnz = T.cb.back();
@@ -1067,7 +1072,7 @@ void Plan::launch_final_table(int l, FinalTable& T, bool ids, const char* stage)
     const double mnnz = (double)J.nnz[l - 2], mnv = (double)J.nv[l - 2], mne = (double)J.ne[l - 2];
     const double nnz1 = (double)J.nnz[l - 1], nv1 = (double)J.nv[l - 1], nf1 = (double)J.nf[l - 1];
     const double ne1 = (double)J.ne[l - 1];
-    reads += 20.0 * mnnz + 4.0 * mnv + (M.blk ? 4.0 * mne : 0.0) + (scheme_ == kSLAK ? 4.0 * mnnz : 0.0) + 16.0 * nv1;
+    reads += (M.rows_x ? 16.0 : 20.0) * mnnz + 4.0 * mnv + (M.blk ? 4.0 * mne : 0.0) + (scheme_ == kSLAK ? 4.0 * mnnz : 0.0) + 16.0 * nv1;
     writes += 16.0 * nv1 + 16.0 * nf1 + 16.0 * ne1;
     if (ids) writes += 16.0 * nnz1 + (J.out_ptr ? 4.0 * (nnz1 + 1) + 16.0 * nnz1 : 0.0);
   }
@@ -1124,6 +1129,7 @@ void Plan::record_level(int l) {
   R.nmpos = L.nmpos;
   R.next_nv = (int32_t)L.next_nv;
   R.next_nnz = (int32_t)L.next_nnz;
+  R.next_rows_x = l < (int)levels_.size() && levels_[l].rows_x ? 1 : 0;
   if (L.segs.size() == 1 && l >= 2) {
     // one mesh: its vertex classes (the original vertices, then the face and
     // edge points of every earlier level) as runs in step with the corners
@@ -1152,7 +1158,7 @@ void Plan::record_level(int l) {
       if (S.dst_ids) writes += 16.0 * snnz;
       if (S.dst_ptr) writes += 4.0 * (snnz + 1) + 16.0 * snnz;  // LAK form of a final level
       if (next && S.next_cb >= 0) {
-        writes += 64.0 * snnz + 4.0 * ((double)S.nv + (double)S.nf + sne) + (L.nbnd ? 4.0 * (snnz + 2.0 * sne) : 0.0) +
+        writes += (R.next_rows_x ? 48.0 : 64.0) * snnz + 4.0 * ((double)S.nv + (double)S.nf + sne) + (L.nbnd ? 4.0 * (snnz + 2.0 * sne) : 0.0) +
                   (scheme_ == kSLAK ? 16.0 * snnz : 0.0);
         if (scheme_ == kSLAK) reads += 4.0 * snnz;
       }

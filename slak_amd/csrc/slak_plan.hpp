@@ -104,6 +104,10 @@ struct LevelPlan {
   // this level launches no face points and no refine or vertex items
   bool pos_fused = false;
   FinalTable pos;
+  // this level's M^T rows are read by k_final alone (no refine or vertex
+  // items run over them): the level before writes only their corners, as
+  // int32 in the row buffer
+  bool rows_x = false;
   SegDesc* d_segs = nullptr;
   int64_t* d_vbase = nullptr;
   int64_t* d_fbase = nullptr;

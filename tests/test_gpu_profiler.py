@@ -27,13 +27,16 @@ def expected_cfg1_lak():
         # mapped SpGEMM row sizing, then edge numbering with the face points
         "spgemm_symbolic_quad": (1, 16 * nnz0 + 8 * nv0),
         "number_facepoints_quad": (1, 12 * nv0 + 20 * nnz0 + 4 * ne0 + 4 * nnz0 + 16 * nv0 + 16 * nf0),
-        # level 1: refine + vertex items, writing level 2's structure
+        # level 1: refine + vertex items, writing level 1's structure; only
+        # the last level's kernel reads it, so its M^T rows are corners alone
+        # (4 refined corners x (8 B (edge, twin) + 4 B row entry))
         "refine_vertex_quad": (1, (20 * nnz0 + 20 * nv0 + 16 * nf0)
-                               + (16 * nnz0 + 16 * ne0 + 16 * nv0 + 64 * nnz0 + 4 * (nv0 + nf0 + ne0))),
+                               + (16 * nnz0 + 16 * ne0 + 16 * nv0 + 48 * nnz0 + 4 * (nv0 + nf0 + ne0))),
         # level 2: positions only, from the input's structure and level 1's positions
         "refine_vertex_pos_quad": (1, (20 * nnz0 + 4 * nv0 + 16 * nv1) + 16 * (nv1 + nf1 + ne1)),
-        # level 3: from level 1's structure and level 2's positions, with quads and LAK form
-        "refine_vertex_final_quad": (1, (20 * nnz1 + 4 * nv1 + 16 * nv2) + 16 * (nv2 + nf2 + ne2)
+        # level 3: from level 1's structure (ids, (edge, twin), row corners)
+        # and level 2's positions, with quads and LAK form
+        "refine_vertex_final_quad": (1, (16 * nnz1 + 4 * nv1 + 16 * nv2) + 16 * (nv2 + nf2 + ne2)
                                      + 16 * nnz2 + 4 * (nnz2 + 1) + 16 * nnz2),
     }
 
