@@ -756,8 +756,9 @@ __device__ __forceinline__ int32_t quad_prev_i(int32_t x) { return __builtin_amd
 
 // entry k of the next level's M^T rows: (corner, in-neighbour), or the corner
 // alone when those rows are read by the last level's kernel only
+template <bool QUAD>
 __device__ __forceinline__ void put_next_row(const RefineView& R, int64_t k, int2 x) {
-  if (R.next_rows_x)
+  if (QUAD && R.next_rows_x)  // (only a quad level writes such rows)
     reinterpret_cast<int32_t*>(R.ntrow)[k] = x.x;
   else
     R.ntrow[k] = x;
@@ -923,7 +924,7 @@ __device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K,
   // face point row: corner 2 of each refined quad of f, in corner order;
   // in-neighbour ep(e1)
   const int32_t nep = nvb + snv + snf;
-  put_next_row(R, ncb + snnz + lt, make_int2(c4 + 2, nep + c.e1));
+  put_next_row<QUAD>(R, ncb + snnz + lt, make_int2(c4 + 2, nep + c.e1));
   bool first;
   if (QUAD) first = (c.t & 3) == 0;
   else first = c.t == R.fptr[c.f];
@@ -940,7 +941,7 @@ __device__ __forceinline__ void corner_store(const RefineView& R, const SegK& K,
       a0 = a1;
       a1 = x;
     }
-    auto q = [&](int k, int2 x) __attribute__((always_inline)) { put_next_row(R, row1 + k, x); };
+    auto q = [&](int k, int2 x) __attribute__((always_inline)) { put_next_row<QUAD>(R, row1 + k, x); };
     if (c.s1 < 0) {
       q(0, a0);
       q(1, a1);
@@ -1079,6 +1080,9 @@ __device__ __forceinline__ void vertex_item(const RefineView& R, int32_t v) {
     corner_nbrs<QUAD>(R.fptr, R.cface, t, f, tn, tp);
     return tp;
   };
+  // the in-half-edges give the refined row's in-neighbours (and the boundary
+  // rule): not needed when the next level's rows are corners alone
+  const bool load_cp = BND || (NEXT && !(QUAD && R.next_rows_x));
   int2 e[kVertexRegs], cp[kVertexRegs], co[kVertexRegs];
   float4 a[kVertexRegs], b[kVertexRegs];
 #pragma unroll
@@ -1089,7 +1093,7 @@ __device__ __forceinline__ void vertex_item(const RefineView& R, int32_t v) {
     if (i < n) {
       a[i] = fpts[face(e[i].x)];
       b[i] = P[e[i].y];
-      if (need_prev) cp[i] = ci[prevc(e[i].x)];
+      if (need_prev && load_cp) cp[i] = ci[prevc(e[i].x)];
       if (need_own) co[i] = ci[e[i].x];
     }
   bool boundary = false;
@@ -1141,12 +1145,12 @@ __device__ __forceinline__ void vertex_item(const RefineView& R, int32_t v) {
 #pragma unroll
   for (int i = 0; i < kVertexRegs; ++i)
     if (i < n) {
-      put_next_row(R, nrow + i, make_int2(ncb + 4 * (e[i].x - cb), nep + clampi(cp[i].x, emax)));
+      put_next_row<QUAD>(R, nrow + i, make_int2(ncb + 4 * (e[i].x - cb), load_cp ? nep + clampi(cp[i].x, emax) : 0));
       if (nbnd) nbnd[neb + pos + i] = co[i].y < 0 ? 1 : 0;
     }
   for (int i = kVertexRegs; i < n; ++i) {
     const int32_t t = row[i].x;
-    put_next_row(R, nrow + i, make_int2(ncb + 4 * (t - cb), nep + clampi(ci[prevc(t)].x, emax)));
+    put_next_row<QUAD>(R, nrow + i, make_int2(ncb + 4 * (t - cb), load_cp ? nep + clampi(ci[prevc(t)].x, emax) : 0));
     if (nbnd) nbnd[neb + pos + i] = ci[t].y < 0 ? 1 : 0;
   }
 }

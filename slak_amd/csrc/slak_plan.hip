@@ -905,7 +905,8 @@ void Plan::prepare() {
   // a level whose meshes all come from k_final runs no refine or vertex
   // items over its structure: only k_final reads its M^T rows, and it needs
   // their corners alone
-  for (size_t i = 1; i < levels_.size(); ++i) levels_[i].rows_x = levels_[i].segs.empty() || levels_[i].pos_fused;
+  for (size_t i = 1; i < levels_.size(); ++i)
+    levels_[i].rows_x = levels_[i - 1].quad && (levels_[i].segs.empty() || levels_[i].pos_fused);
   upload_tables(0, (int)levels_.size(), staging_all_);
   prepared_ = true;
 }
