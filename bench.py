@@ -84,6 +84,23 @@ def stage_roofline(prof, stage=None):
     }
 
 
+def output_level_roofline(prof):
+    """The several-mesh output level as one unit: k_final (refine_vertex_final_quad)
+    and the LAK-form launch that follows it (lak_form), their algorithmic
+    bytes over their summed HIP-event durations. None when no LAK form is
+    written separately (SLAK, or one mesh)."""
+    st = {s["name"]: s for s in prof["stages"]}
+    if CFG4_STAGE not in st or "lak_form" not in st:
+        return None
+    a, b = st[CFG4_STAGE], st["lak_form"]
+    n = a["launches"]
+    byts = (a["bytes"] + b["bytes"]) / n
+    ms = (a["total_ms"] + b["total_ms"]) / n
+    gbps = byts / (ms * 1e-3) / 1e9
+    return {"stages": [CFG4_STAGE, "lak_form"], "bytes_per_launch": int(byts), "us_per_launch": round(ms * 1e3, 2),
+            "achieved": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+
+
 def stage_table(prof):
     return [{"name": s["name"], "launches": s["launches"], "us": round(s["total_ms"] * 1e3, 1),
              "bytes": int(s["bytes"]), "gbps": round(s["bytes"] / (s["total_ms"] * 1e-3) / 1e9, 1)}
@@ -298,8 +315,9 @@ STAGE_KERNELS = {
     "facepoints_quad": r"k_faces<true, true, false>",
     "facepoints_poly": r"k_faces<false, true, false>",
     "gather": r"k_gather",
-    "count_quad": r"k_faces<true, false, true>",
-    "count_poly": r"k_faces<false, false, true>",
+    "count_quad": r"k_count<true>",
+    "count_poly": r"k_count<false>",
+    "lak_form": r"k_lak_form",
     "transpose_quad": r"k_transpose<true>",
     "transpose_poly": r"k_transpose<false>",
     "spgemm_symbolic_quad": r"k_symbolic<true, {s},",
@@ -622,6 +640,9 @@ def main():
     secs = B["secs"]
     value = B["faces"] * args.steps / secs
     roof = add_traffic(stage_roofline(B["prof"], CFG4_STAGE), traffic, "cfg4", args.scheme, True)
+    olr = output_level_roofline(B["prof"])
+    if olr is not None:
+        roof["output_level_with_lak_form"] = olr
     floor_b = sum(io_floor_bytes(h, lv, lak_form) for h, lv in zip(spec.meshes, spec.levels))
     line = {
         "metric": METRIC,

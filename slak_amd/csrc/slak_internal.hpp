@@ -84,6 +84,9 @@ void launch_level(bool quad, int scheme, const RefineView& R, hipStream_t st);
 // last level of one mesh or several (V.segs) from the quad mesh two levels
 // back; ids: also the output quads and LAK form (false: positions only)
 void launch_final(int scheme, const FinalView& V, bool ids, hipStream_t st);
+// the LAK form (ptr, vals) of a several-mesh output level, after its k_final
+// launch (which leaves it out)
+void launch_final_lak_form(const FinalView& V, hipStream_t st);
 // homogeneous-mesh conversions: per-face reorder of (ids, vals) by vals or by
 // ids (faces by ptr, or stride when ptr is null; 3 or 4 corners); fills of
 // stride ptr / 1-based positions

@@ -1599,7 +1599,9 @@ __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int3
   }
   // LAK form of the block's 4 x 256 output quads (createQuadColPtr /
   // createQuadVals): every int4 of vals is (1, 2, 3, 4) and int4 q of ptr
-  // is 16q + (0, 4, 8, 12), so consecutive lanes take consecutive pieces
+  // is 16q + (0, 4, 8, 12), so consecutive lanes take consecutive pieces.
+  // Several meshes: written by k_lak_form after this launch instead.
+  if (MULTI) return;
   if (S.out_vals) {
     int32_t* __restrict__ vals = S.out_vals + 16 * (int64_t)t0;
 #pragma unroll
@@ -1614,6 +1616,38 @@ __global__ void __launch_bounds__(kBlock) k_final(FinalView V, int32_t nbc, int3
   }
 }
 
+
+// The LAK form of the output quads of a multi-mesh k_final launch
+// (createQuadColPtr / createQuadVals, LAKCatmullClarkKernels.cuh:396-431),
+// from the same corner units: a block writes the 1024 quads of its unit's
+// 256 corners, vals as int4 (1, 2, 3, 4), ptr as int4 16t + (0, 4, 8, 12).
+// A pure streaming write, apart from the gathers and scattered stores of
+// k_final; each block takes every gridDim.x-th unit (a block per unit spends
+// more on dispatch than on its 20 KB of stores).
+__global__ void __launch_bounds__(kBlock) k_lak_form(FinalView V) {
+  for (int64_t u = block_index(); u < V.nunits; u += gridDim.x) {
+    const int4 U = V.units[u];
+    if (U.w) continue;  // vertex unit
+    const FinalSeg& Sg = V.segs[U.x];
+    int32_t* const out_vals = to_global(Sg.out_vals);
+    int32_t* const out_ptr = to_global(Sg.out_ptr);
+    const int32_t t0 = U.y, cnt = U.z, nnz = Sg.nnz;
+    const int32_t t = t0 + (int32_t)threadIdx.x;
+    const int nq = 4 * cnt;
+    if (out_vals) {
+      int32_t* __restrict__ vals = out_vals + 16 * (int64_t)t0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = j * kBlock + threadIdx.x;
+        if (i < nq) store_nt(vals + 4 * i, 1, 2, 3, 4);
+      }
+    }
+    if (out_ptr) {
+      if ((int)threadIdx.x < cnt) store_nt(out_ptr + 4 * (int64_t)t, 16 * t, 16 * t + 4, 16 * t + 8, 16 * t + 12);
+      if (t == nnz - 1) store_nt(out_ptr + 4 * (int64_t)nnz, 16 * nnz);
+    }
+  }
+}
 
 // ---------------------------------------------------------------------------
 // reduced / unreduced conversions of homogeneous meshes (GPU::CSCMesh::reduce /
@@ -2194,6 +2228,13 @@ void launch_final(int scheme, const FinalView& V0, bool ids, hipStream_t st) {
 #undef SLAK_FINAL
 }
 
+
+constexpr int64_t kLakFormBlocks = 4096;  // 16 per CU
+
+void launch_final_lak_form(const FinalView& V, hipStream_t st) {
+  if (!V.segs || !V.units || V.nunits <= 0) throw Error(5, "final level: no work units");  // SLAK_ERR_INTERNAL
+  k_lak_form<<<(unsigned)std::min<int64_t>(V.nunits, kLakFormBlocks), kBlock, 0, st>>>(V);
+}
 
 void launch_sort_faces(bool by_vals, const int32_t* ptr, int32_t stride, int32_t nf, int64_t nnz, int32_t* ids,
                        int32_t* vals, int* flags, hipStream_t st) {

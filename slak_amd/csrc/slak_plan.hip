@@ -1067,7 +1067,9 @@ void Plan::launch_final_table(int l, FinalTable& T, bool ids, const char* stage)
   // offsets of M (boundary scan, SLAK positions) and the positions of M1;
   // writes every output position and, for the output level, the quads (and
   // the LAK form)
-  double reads = 0, writes = 0;
+  // (several meshes: the LAK form is written by a launch of its own after it)
+  const bool multi = final_table(T);
+  double reads = 0, writes = 0, lak_form = 0;
   for (int j : T.jobs) {
     const MeshJob& J = jobs_[j];
     const double mnnz = (double)J.nnz[l - 2], mnv = (double)J.nv[l - 2], mne = (double)J.ne[l - 2];
@@ -1075,10 +1077,19 @@ void Plan::launch_final_table(int l, FinalTable& T, bool ids, const char* stage)
     const double ne1 = (double)J.ne[l - 1];
     reads += (M.rows_x ? 16.0 : 20.0) * mnnz + 4.0 * mnv + (M.blk ? 4.0 * mne : 0.0) + (scheme_ == kSLAK ? 4.0 * mnnz : 0.0) + 16.0 * nv1;
     writes += 16.0 * nv1 + 16.0 * nf1 + 16.0 * ne1;
-    if (ids) writes += 16.0 * nnz1 + (J.out_ptr ? 4.0 * (nnz1 + 1) + 16.0 * nnz1 : 0.0);
+    if (ids) {
+      writes += 16.0 * nnz1;
+      if (J.out_ptr) (multi ? lak_form : writes) += 4.0 * (nnz1 + 1) + 16.0 * nnz1;
+    }
   }
-  ProfScope ps(st_, stage, reads + writes);
-  launch_final(scheme_, V, ids, st_);
+  {
+    ProfScope ps(st_, stage, reads + writes);
+    launch_final(scheme_, V, ids, st_);
+  }
+  if (multi && lak_form > 0) {
+    ProfScope ps(st_, "lak_form", lak_form);
+    launch_final_lak_form(V, st_);
+  }
 }
 
 void Plan::record_level(int l) {
