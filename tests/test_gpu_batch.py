@@ -190,6 +190,39 @@ def test_config4_sample_batch_matches_per_mesh():
             d.freeAndReset()
 
 
+@pytest.mark.parametrize("seed", [21, 22, 23, 24])
+def test_random_batches_equal_single_mesh_calls(seed):
+    """Random compositions (tori, polygonal tori with pentagons, open grids,
+    cubes, tetrahedra; depths 0-4), both schemes, eager and graph: the
+    several-mesh paths (work-unit tables, LAK form in its own launch, rows
+    read by the last level alone written as corners) give each mesh its
+    single-mesh result bit for bit."""
+    rng = np.random.default_rng(seed)
+    hosts, _ = mixed_batch(int(rng.integers(12, 36)), seed=seed)
+    levels = [int(rng.integers(0, 5)) for _ in hosts]
+    for scheme in (LAK, SLAK):
+        dev = [MeshInfo.from_host(h) for h in hosts]
+        try:
+            with Batch(dev, levels, scheme) as b:
+                b.run(use_graph=bool(seed % 2))
+                b.run(use_graph=bool(seed % 2))
+                for i, (h, lv) in enumerate(zip(hosts, levels)):
+                    r = b.result(i)
+                    if h.type == g.QUAD and lv == 0:
+                        assert r.ids == 0
+                        continue
+                    ids, verts, ptr, vals = fetch(r)
+                    single, _ = subdivide_host(h, lv, scheme)
+                    np.testing.assert_array_equal(ids, single.ids, err_msg=f"mesh {i} level {lv}")
+                    np.testing.assert_array_equal(verts, single.verts, err_msg=f"mesh {i} level {lv}")
+                    if scheme == LAK:
+                        np.testing.assert_array_equal(ptr, single.ptr, err_msg=f"mesh {i} level {lv}")
+                        np.testing.assert_array_equal(vals, single.vals, err_msg=f"mesh {i} level {lv}")
+        finally:
+            for d in dev:
+                d.freeAndReset()
+
+
 def test_batch_reports_the_invalid_mesh():
     hosts = [g.cube(), g.tetrahedron(), g.torus(6, 5)]
     bad = g.HostMesh(hosts[2].type, hosts[2].ptr, hosts[2].ids.copy(), hosts[2].vals, hosts[2].verts)
