@@ -21,6 +21,13 @@
 //                 (LAKCatmullClarkImpl.cu:200-235) and boundary rule
 //                 (LAKCatmullClarkKernels.cuh:464-511); the vertex rows of the
 //                 refined M^T
+// Last two levels of a quad chain:
+//   final         from the quad mesh M two levels back: the output quads,
+//                 edge, face and vertex points of the level (or its positions
+//                 only, for the level before it); several meshes per launch
+//                 through work-unit tables
+//   LAK form      several meshes: the output level's ptr / vals in a launch
+//                 of their own (createQuadColPtr / createQuadVals)
 // All kernels are pull-based and atomic-free except the integer valence
 // count and transpose slots, whose result does not depend on atomic order,
 // so every output is bitwise deterministic.
